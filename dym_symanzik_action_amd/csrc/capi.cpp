@@ -1,0 +1,524 @@
+// capi.cpp -- the C ABI of libdsa.so (include/dsa.h): group and chain handles,
+// device memory, and the sweep / measurement drivers.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dsa.h"
+#include "dsa_math.hpp"
+#include "group.hpp"
+#include "kernels.hpp"
+#include "schedule.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail(DSA_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DeviceTables {
+  uint16_t *mult = nullptr, *inv = nullptr, *pool = nullptr;
+  double *retr = nullptr, *imtr = nullptr;
+};
+
+}  // namespace
+
+struct dsa_group {
+  dsa::Group g;
+  std::mutex m;
+  std::map<int, DeviceTables> dev;
+};
+
+struct dsa_chain {
+  dsa_group *grp = nullptr;
+  DeviceTables dt;
+  dsa::Tables T{};
+  dsa::Geom G{};
+  dsa_chain_params p{};
+  dsa::Schedule S;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint16_t *d_links = nullptr;
+  uint32_t *d_sites = nullptr;
+  uint32_t *d_mstate = nullptr;
+  unsigned long long *d_acc = nullptr;
+  int32_t *d_aos = nullptr;
+  uint16_t *d_lines = nullptr;
+  size_t lines_cap = 0;
+  uint32_t lines_t = 0, lines_x = 0;
+  bool lines_valid = false;
+  double *d_partials = nullptr;
+  size_t partials_cap = 0;
+  double *d_out = nullptr;
+  size_t out_cap = 0;
+  uint64_t sweep = 0, hit = 0;
+  double last_ms = 0;
+  int max_blocks = 2048;
+};
+
+namespace {
+
+int upload_tables(dsa_group *g, int device, DeviceTables &out) {
+  std::lock_guard<std::mutex> lk(g->m);
+  auto it = g->dev.find(device);
+  if (it != g->dev.end()) {
+    out = it->second;
+    return 0;
+  }
+  DeviceTables t;
+  const dsa::Group &G = g->g;
+  const size_t P = G.P;
+  HIP_TRY(hipMalloc(&t.mult, P * P * sizeof(uint16_t)));
+  HIP_TRY(hipMalloc(&t.inv, P * sizeof(uint16_t)));
+  HIP_TRY(hipMalloc(&t.pool, G.pool.size() * sizeof(uint16_t)));
+  HIP_TRY(hipMalloc(&t.retr, P * sizeof(double)));
+  HIP_TRY(hipMalloc(&t.imtr, P * sizeof(double)));
+  HIP_TRY(hipMemcpy(t.mult, G.mult.data(), P * P * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.inv, G.inv.data(), P * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.pool, G.pool.data(), G.pool.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.retr, G.retr.data(), P * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(t.imtr, G.imtr.data(), P * sizeof(double), hipMemcpyHostToDevice));
+  g->dev[device] = t;
+  out = t;
+  return 0;
+}
+
+void free_tables(DeviceTables &t) {
+  if (t.mult) (void)hipFree(t.mult);
+  if (t.inv) (void)hipFree(t.inv);
+  if (t.pool) (void)hipFree(t.pool);
+  if (t.retr) (void)hipFree(t.retr);
+  if (t.imtr) (void)hipFree(t.imtr);
+  t = DeviceTables();
+}
+
+int ensure_lines(dsa_chain *c, uint32_t nt, uint32_t nx) {
+  if (c->lines_valid && c->lines_t == nt && c->lines_x == nx) return 0;
+  const size_t rows = (size_t)nt + (size_t)(c->G.D - 1) * nx;
+  const size_t need = rows * c->G.V * sizeof(uint16_t);
+  if (need > c->lines_cap) {
+    if (c->d_lines) HIP_TRY(hipFree(c->d_lines));
+    c->d_lines = nullptr;
+    c->lines_cap = 0;
+    HIP_TRY(hipMalloc(&c->d_lines, need));
+    c->lines_cap = need;
+  }
+  HIP_TRY(dsa::launch_lines(c->d_lines, c->d_links, c->T, c->G, nt, nx, c->stream));
+  c->lines_t = nt;
+  c->lines_x = nx;
+  c->lines_valid = true;
+  return 0;
+}
+
+int set_device(dsa_chain *c) {
+  HIP_TRY(hipSetDevice(c->device));
+  return 0;
+}
+
+int check_links(const dsa_chain *c, const int32_t *a) {
+  const size_t n = (size_t)c->G.V * c->G.D;
+  for (size_t l = 0; l < n; ++l)
+    if (a[l] < 0 || (uint32_t)a[l] >= c->T.P)
+      return fail(DSA_E_ARG, "link " + std::to_string(l) + " holds " + std::to_string(a[l]) + ", outside [0, P)");
+  return 0;
+}
+
+int upload_links(dsa_chain *c, const int32_t *a) {
+  const size_t n = (size_t)c->G.V * c->G.D;
+  HIP_TRY(hipMemcpyAsync(c->d_aos, a, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(dsa::launch_pack(c->d_links, c->d_aos, c->G, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->lines_valid = false;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dsa_last_error(void) { return g_err.c_str(); }
+const char *dsa_version(void) { return "dsa-mi355x 0.1 (gfx950)"; }
+
+int dsa_group_load(const char *path, dsa_group **out) {
+  if (!out) return fail(DSA_E_ARG, "dsa_group_load: out is NULL");
+  *out = nullptr;
+  auto *g = new dsa_group();
+  std::string err = dsa::load_group(path, g->g);
+  if (!err.empty()) {
+    delete g;
+    return fail(err.rfind("cannot open", 0) == 0 ? DSA_E_IO : DSA_E_FORMAT, err);
+  }
+  *out = g;
+  return 0;
+}
+
+int dsa_group_info_get(const dsa_group *g, dsa_group_info *info) {
+  if (!g || !info) return fail(DSA_E_ARG, "dsa_group_info_get: NULL argument");
+  info->P = g->g.P;
+  info->id = g->g.id;
+  info->npool = (uint32_t)g->g.pool.size();
+  info->min_retr = g->g.min_retr;
+  info->nn_retr = g->g.nn_retr;
+  return 0;
+}
+
+int dsa_group_tables(const dsa_group *g, int32_t *mult, int32_t *inv, double *retr, double *imtr, int32_t *pool) {
+  if (!g) return fail(DSA_E_ARG, "dsa_group_tables: NULL group");
+  const size_t P = g->g.P;
+  if (mult)
+    for (size_t n = 0; n < P * P; ++n) mult[n] = g->g.mult[n];
+  if (inv)
+    for (size_t n = 0; n < P; ++n) inv[n] = g->g.inv[n];
+  if (retr) memcpy(retr, g->g.retr.data(), P * sizeof(double));
+  if (imtr) memcpy(imtr, g->g.imtr.data(), P * sizeof(double));
+  if (pool)
+    for (size_t n = 0; n < g->g.pool.size(); ++n) pool[n] = g->g.pool[n];
+  return 0;
+}
+
+void dsa_group_free(dsa_group *g) {
+  if (!g) return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto &kv : g->dev) {
+    (void)hipSetDevice(kv.first);
+    free_tables(kv.second);
+  }
+  (void)hipSetDevice(cur);
+  delete g;
+}
+
+int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain **out) {
+  if (!gc || !p || !out) return fail(DSA_E_ARG, "dsa_chain_create: NULL argument");
+  *out = nullptr;
+  dsa_group *g = const_cast<dsa_group *>(gc);
+  if (p->D < 2 || p->D > (uint32_t)dsa::kDMax)
+    return fail(DSA_E_RANGE, "D = " + std::to_string(p->D) + " outside the supported 2..5");
+  if (p->Nt < 1 || p->Nx < 1) return fail(DSA_E_ARG, "Nt and Nx must be positive");
+  uint64_t V = p->Nt;
+  for (uint32_t d = 1; d < p->D; ++d) V *= p->Nx;
+  if (V * p->D >= (1ull << 31)) return fail(DSA_E_RANGE, "V*D must be below 2^31 links");
+  if (p->rng != DSA_RNG_PHILOX && p->rng != DSA_RNG_MINSTD) return fail(DSA_E_ARG, "unknown rng");
+  if (p->order != DSA_ORDER_COLOUR && p->order != DSA_ORDER_REFERENCE) return fail(DSA_E_ARG, "unknown order");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DSA_E_DEVICE, "no HIP device visible");
+
+  auto *c = new dsa_chain();
+  c->grp = g;
+  c->p = *p;
+  c->G = dsa::make_geom(p->D, p->Nt, p->Nx);
+  auto bail = [&](int code) {
+    dsa_chain_destroy(c);
+    return code;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(DSA_E_DEVICE, "hipGetDevice failed"));
+  int rc = upload_tables(g, c->device, c->dt);
+  if (rc) return bail(rc);
+  c->T.mult = c->dt.mult;
+  c->T.inv = c->dt.inv;
+  c->T.retr = c->dt.retr;
+  c->T.imtr = c->dt.imtr;
+  c->T.pool = c->dt.pool;
+  c->T.P = g->g.P;
+  c->T.npool = (uint32_t)g->g.pool.size();
+  c->T.id = (uint32_t)g->g.id;
+  if (dsa::sweep_lds_bytes(c->T) > 65536)
+    return bail(fail(DSA_E_RANGE, "group order " + std::to_string(c->T.P) + " needs more than 64 KB of LDS tables"));
+
+  std::string err = p->order == DSA_ORDER_COLOUR ? dsa::build_colour_schedule(c->G, c->S)
+                                                 : dsa::build_reference_schedule(c->G, c->S);
+  if (!err.empty()) return bail(fail(DSA_E_STATE, err));
+
+  const size_t nl = (size_t)c->G.V * c->G.D;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return bail(fail(DSA_E_DEVICE, "stream/event creation failed"));
+  if (hipMalloc(&c->d_links, nl * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&c->d_aos, nl * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->d_sites, c->S.sites.size() * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->d_acc, sizeof(unsigned long long)) != hipSuccess)
+    return bail(fail(DSA_E_DEVICE, "device allocation failed"));
+  if (hipMemcpy(c->d_sites, c->S.sites.data(), c->S.sites.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemset(c->d_acc, 0, sizeof(unsigned long long)) != hipSuccess)
+    return bail(fail(DSA_E_DEVICE, "device upload failed"));
+
+  const bool cold = p->start == DSA_START_COLD;
+  if (p->rng == DSA_RNG_MINSTD) {
+    // dym-mod-metro.cpp:227-228, 253-255: per-site engines seeded iseed+i; hot
+    // links drawn in link order from site 0's engine (host: it is sequential).
+    std::vector<uint32_t> ms(c->G.V);
+    for (uint32_t i = 0; i < c->G.V; ++i) {
+      const uint64_t s = (uint64_t)(int64_t)(int32_t)((uint32_t)p->seed + i);
+      uint64_t x = s % 2147483647ull;
+      ms[i] = (uint32_t)(x == 0 ? 1 : x);
+    }
+    std::vector<int32_t> a(nl, g->g.id);
+    if (!cold) {
+      const uint64_t urngrange = 2147483645ull, uerange = g->g.P;
+      const uint64_t scaling = urngrange / uerange, past = uerange * scaling;
+      for (size_t l = 0; l < nl; ++l) {
+        uint64_t r;
+        do r = (uint64_t)dsa::minstd_next(ms[0]) - 1;
+        while (r >= past);
+        a[l] = (int32_t)(r / scaling);
+      }
+    }
+    if (hipMalloc(&c->d_mstate, c->G.V * sizeof(uint32_t)) != hipSuccess ||
+        hipMemcpy(c->d_mstate, ms.data(), c->G.V * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(DSA_E_DEVICE, "minstd state upload failed"));
+    rc = upload_links(c, a.data());
+    if (rc) return bail(rc);
+  } else {
+    hipError_t e = cold ? dsa::launch_fill(c->d_links, c->G, (uint16_t)g->g.id, c->stream)
+                        : dsa::launch_init_philox(c->d_links, c->G, c->T.P, (uint32_t)p->seed, c->stream);
+    if (e != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+      return bail(fail(DSA_E_DEVICE, "hot/cold start kernel failed"));
+  }
+  *out = c;
+  return 0;
+}
+
+void dsa_chain_destroy(dsa_chain *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *ptr : {(void *)c->d_links, (void *)c->d_sites, (void *)c->d_mstate, (void *)c->d_acc,
+                    (void *)c->d_aos, (void *)c->d_lines, (void *)c->d_partials, (void *)c->d_out})
+    if (ptr) (void)hipFree(ptr);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_sweep: NULL chain");
+  if (set_device(c)) return DSA_E_DEVICE;
+  dsa::SweepArgs A{};
+  A.links = c->d_links;
+  A.mstate = c->d_mstate;
+  A.acc = c->d_acc;
+  A.T = c->T;
+  A.G = c->G;
+  A.rng = c->p.rng == DSA_RNG_MINSTD ? dsa::kRngMinstd : dsa::kRngPhilox;
+  A.beta0 = c->p.beta0;
+  A.beta1 = c->p.beta1;
+  A.beta2 = c->p.beta2;
+  A.seed = (uint32_t)c->p.seed;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t it = 0; it < n; ++it) {
+    A.sweep_lo = (uint32_t)c->sweep;
+    A.sweep_hi = (uint32_t)(c->sweep >> 32);
+    for (const dsa::Pass &ps : c->S.passes) {
+      A.mu = ps.mu;
+      A.sites = c->d_sites + ps.offset;
+      A.count = ps.count;
+      HIP_TRY(dsa::launch_sweep(A, c->max_blocks, c->stream));
+      c->hit += (uint64_t)ps.count * dsa::kNHit;
+    }
+    c->sweep++;
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  if (n) c->lines_valid = false;
+  return 0;
+}
+
+int dsa_chain_counters(dsa_chain *c, uint64_t *hit, uint64_t *acc) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_counters: NULL chain");
+  if (set_device(c)) return DSA_E_DEVICE;
+  unsigned long long a = 0;
+  HIP_TRY(hipMemcpy(&a, c->d_acc, sizeof a, hipMemcpyDeviceToHost));
+  if (hit) *hit = c->hit;
+  if (acc) *acc = a;
+  return 0;
+}
+
+int dsa_chain_get_sweep(dsa_chain *c, uint64_t *sweep) {
+  if (!c || !sweep) return fail(DSA_E_ARG, "dsa_chain_get_sweep: NULL argument");
+  *sweep = c->sweep;
+  return 0;
+}
+
+int dsa_chain_set_sweep(dsa_chain *c, uint64_t sweep) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_set_sweep: NULL chain");
+  c->sweep = sweep;
+  return 0;
+}
+
+int dsa_chain_measure(dsa_chain *c, uint32_t wk, uint32_t wl, int32_t sum_mode, double *out) {
+  if (!c || !out) return fail(DSA_E_ARG, "dsa_chain_measure: NULL argument");
+  if (wk > c->p.Nx || wl > c->p.Nt) return fail(DSA_E_ARG, "Wilson grid larger than Nx x Nt");
+  if ((wk == 0) != (wl == 0)) return fail(DSA_E_ARG, "Wilson grid must be empty or at least 1x1");
+  if (sum_mode != DSA_SUM_TREE && sum_mode != DSA_SUM_ORDERED) return fail(DSA_E_ARG, "unknown sum mode");
+  if (set_device(c)) return DSA_E_DEVICE;
+  if (wk && wl) {
+    int rc = ensure_lines(c, wl, wk);
+    if (rc) return rc;
+  }
+  const uint32_t nobs = 4 + wk * wl;
+  if (c->out_cap < nobs) {
+    if (c->d_out) HIP_TRY(hipFree(c->d_out));
+    c->d_out = nullptr;
+    HIP_TRY(hipMalloc(&c->d_out, nobs * sizeof(double)));
+    c->out_cap = nobs;
+  }
+  const uint16_t *lines = (wk && wl) ? c->d_lines : nullptr;
+  if (sum_mode == DSA_SUM_TREE) {
+    const size_t need = (size_t)nobs * dsa::obs_blocks(c->G);
+    if (c->partials_cap < need) {
+      if (c->d_partials) HIP_TRY(hipFree(c->d_partials));
+      c->d_partials = nullptr;
+      HIP_TRY(hipMalloc(&c->d_partials, need * sizeof(double)));
+      c->partials_cap = need;
+    }
+    HIP_TRY(dsa::launch_measure_tree(c->d_out, c->d_partials, c->d_links, lines, c->T, c->G, wk, wl, wl, wk,
+                                     c->stream));
+  } else {
+    HIP_TRY(dsa::launch_measure_ordered(c->d_out, c->d_links, lines, c->T, c->G, wk, wl, wl, wk, c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(out, c->d_out, nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int dsa_chain_get_links(dsa_chain *c, int32_t *a) {
+  if (!c || !a) return fail(DSA_E_ARG, "dsa_chain_get_links: NULL argument");
+  if (set_device(c)) return DSA_E_DEVICE;
+  const size_t n = (size_t)c->G.V * c->G.D;
+  HIP_TRY(dsa::launch_unpack(c->d_aos, c->d_links, c->G, c->stream));
+  HIP_TRY(hipMemcpyAsync(a, c->d_aos, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int dsa_chain_set_links(dsa_chain *c, const int32_t *a) {
+  if (!c || !a) return fail(DSA_E_ARG, "dsa_chain_set_links: NULL argument");
+  if (check_links(c, a)) return DSA_E_ARG;
+  if (set_device(c)) return DSA_E_DEVICE;
+  return upload_links(c, a);
+}
+
+int dsa_chain_get_rng_state(dsa_chain *c, uint32_t *state) {
+  if (!c || !state) return fail(DSA_E_ARG, "dsa_chain_get_rng_state: NULL argument");
+  if (!c->d_mstate) return fail(DSA_E_STATE, "chain does not use the minstd streams");
+  if (set_device(c)) return DSA_E_DEVICE;
+  HIP_TRY(hipMemcpy(state, c->d_mstate, c->G.V * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dsa_chain_set_rng_state(dsa_chain *c, const uint32_t *state) {
+  if (!c || !state) return fail(DSA_E_ARG, "dsa_chain_set_rng_state: NULL argument");
+  if (!c->d_mstate) return fail(DSA_E_STATE, "chain does not use the minstd streams");
+  for (uint32_t i = 0; i < c->G.V; ++i)
+    if (state[i] == 0 || state[i] >= 2147483647u) return fail(DSA_E_ARG, "minstd state outside [1, 2^31-2]");
+  if (set_device(c)) return DSA_E_DEVICE;
+  HIP_TRY(hipMemcpy(c->d_mstate, state, c->G.V * sizeof(uint32_t), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int dsa_chain_staples(dsa_chain *c, int32_t *st, int32_t *re) {
+  if (!c || !st || !re) return fail(DSA_E_ARG, "dsa_chain_staples: NULL argument");
+  if (set_device(c)) return DSA_E_DEVICE;
+  const size_t nl = (size_t)c->G.V * c->G.D, ns = 2 * (c->G.D - 1), nr = 6 * (c->G.D - 1);
+  int32_t *dst = nullptr, *dre = nullptr;
+  HIP_TRY(hipMalloc(&dst, nl * ns * sizeof(int32_t)));
+  if (hipMalloc(&dre, nl * nr * sizeof(int32_t)) != hipSuccess) {
+    (void)hipFree(dst);
+    return fail(DSA_E_DEVICE, "allocation failed");
+  }
+  hipError_t e = dsa::launch_staples(dst, dre, c->d_links, c->T, c->G, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(st, dst, nl * ns * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(re, dre, nl * nr * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dst);
+  (void)hipFree(dre);
+  if (e != hipSuccess) return fail(DSA_E_DEVICE, std::string("staples: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int dsa_chain_loops(dsa_chain *c, int32_t kind, uint32_t k, uint32_t l, int32_t *out) {
+  if (!c || !out) return fail(DSA_E_ARG, "dsa_chain_loops: NULL argument");
+  if (kind < DSA_LOOP_PLAQ || kind > DSA_LOOP_WILSON) return fail(DSA_E_ARG, "unknown loop kind");
+  if (kind == DSA_LOOP_WILSON && (k >= c->p.Nx || l >= c->p.Nt)) return fail(DSA_E_ARG, "Wilson loop too large");
+  if (set_device(c)) return DSA_E_DEVICE;
+  const uint32_t D = c->G.D, V = c->G.V;
+  size_t n = kind == DSA_LOOP_PLAQ ? (size_t)V * D * (D - 1) / 2
+             : kind == DSA_LOOP_RECT ? (size_t)V * D * (D - 1)
+             : kind == DSA_LOOP_POLY ? (size_t)V / c->p.Nt
+                                     : (size_t)V * (D - 1);
+  if (kind == DSA_LOOP_WILSON) {
+    int rc = ensure_lines(c, l + 1, k + 1);
+    if (rc) return rc;
+  }
+  int32_t *d = nullptr;
+  HIP_TRY(hipMalloc(&d, n * sizeof(int32_t)));
+  hipError_t e = dsa::launch_loops(d, kind, (int)k, (int)l, c->d_links, c->d_lines, c->T, c->G, c->lines_t,
+                                   c->lines_x, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(DSA_E_DEVICE, std::string("loops: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int dsa_chain_schedule_info(dsa_chain *c, uint32_t *npasses, uint32_t *ncolours) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_schedule_info: NULL chain");
+  if (npasses) *npasses = (uint32_t)c->S.passes.size();
+  if (ncolours)
+    for (int d = 0; d < c->G.D; ++d) ncolours[d] = c->S.ncol[d];
+  return 0;
+}
+
+int dsa_schedule_query(uint32_t D, uint32_t Nt, uint32_t Nx, int32_t order, uint32_t *npasses, uint32_t *nsites,
+                       uint32_t *pass_mu, uint32_t *pass_count, uint32_t *sites) {
+  if (!npasses || !nsites) return fail(DSA_E_ARG, "dsa_schedule_query: NULL counts");
+  if (D < 2 || D > (uint32_t)dsa::kDMax || Nt < 1 || Nx < 1) return fail(DSA_E_RANGE, "unsupported lattice shape");
+  uint64_t V = Nt;
+  for (uint32_t d = 1; d < D; ++d) V *= Nx;
+  if (V * D >= (1ull << 31)) return fail(DSA_E_RANGE, "V*D must be below 2^31 links");
+  dsa::Geom G = dsa::make_geom(D, Nt, Nx);
+  dsa::Schedule S;
+  std::string err = order == DSA_ORDER_REFERENCE ? dsa::build_reference_schedule(G, S)
+                                                 : dsa::build_colour_schedule(G, S);
+  if (!err.empty()) return fail(DSA_E_STATE, err);
+  const bool fill = pass_mu && pass_count && sites;
+  if (fill) {
+    if (*npasses < S.passes.size() || *nsites < S.sites.size()) return fail(DSA_E_ARG, "output arrays too small");
+    for (size_t q = 0; q < S.passes.size(); ++q) {
+      pass_mu[q] = (uint32_t)S.passes[q].mu;
+      pass_count[q] = S.passes[q].count;
+    }
+    memcpy(sites, S.sites.data(), S.sites.size() * sizeof(uint32_t));
+  }
+  *npasses = (uint32_t)S.passes.size();
+  *nsites = (uint32_t)S.sites.size();
+  return 0;
+}
+
+int dsa_chain_last_sweep_ms(dsa_chain *c, double *ms) {
+  if (!c || !ms) return fail(DSA_E_ARG, "dsa_chain_last_sweep_ms: NULL argument");
+  *ms = c->last_ms;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+"""ctypes binding of libdsa.so and the Python-side mirror of the reference API.
+
+The reference drives this path through globals (group.h:8-16, lattice.h:7-16)
+and free functions (load_group, update, plaquette, polyakov, getpoly, wilson).
+Here the same operations are methods on handles; argument meanings and the
+element/enumeration orders are the reference's.  Errors raise DSAError with the
+library's message instead of aborting.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libdsa.so")
+GROUP_DIR = os.path.join(_HERE, "groups")
+
+DSA_RNG = {"philox": 0, "minstd": 1}
+DSA_ORDER = {"colour": 0, "reference": 1}
+DSA_START = {"hot": 0, "cold": 1}
+DSA_SUM = {"tree": 0, "ordered": 1}
+LOOP_KIND = {"plaq": 0, "rect": 1, "poly": 2, "wilson": 3}
+
+
+class DSAError(RuntimeError):
+    """A libdsa call returned a negative status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dsa error {code}: {msg}")
+        self.code = code
+
+
+class GroupInfo(C.Structure):
+    _fields_ = [
+        ("P", C.c_uint32),
+        ("id", C.c_int32),
+        ("npool", C.c_uint32),
+        ("min_retr", C.c_double),
+        ("nn_retr", C.c_double),
+    ]
+
+
+class ChainParams(C.Structure):
+    _fields_ = [
+        ("D", C.c_uint32),
+        ("Nt", C.c_uint32),
+        ("Nx", C.c_uint32),
+        ("beta0", C.c_double),
+        ("beta1", C.c_double),
+        ("beta2", C.c_double),
+        ("seed", C.c_int32),
+        ("start", C.c_int32),
+        ("rng", C.c_int32),
+        ("order", C.c_int32),
+    ]
+
+
+_lib: Optional[C.CDLL] = None
+
+_SIGS = {
+    "dsa_group_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "dsa_group_info_get": (C.c_int, [C.c_void_p, C.POINTER(GroupInfo)]),
+    "dsa_group_tables": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dsa_group_free": (None, [C.c_void_p]),
+    "dsa_chain_create": (C.c_int, [C.c_void_p, C.POINTER(ChainParams), C.POINTER(C.c_void_p)]),
+    "dsa_chain_destroy": (None, [C.c_void_p]),
+    "dsa_chain_sweep": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "dsa_chain_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "dsa_chain_get_sweep": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "dsa_chain_set_sweep": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "dsa_chain_measure": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p]),
+    "dsa_chain_get_links": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dsa_chain_set_links": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dsa_chain_get_rng_state": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dsa_chain_set_rng_state": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dsa_chain_staples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dsa_chain_loops": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "dsa_chain_schedule_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p]),
+    "dsa_chain_last_sweep_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "dsa_schedule_query": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dsa_last_error": (C.c_char_p, []),
+    "dsa_version": (C.c_char_p, []),
+}
+
+
+def lib() -> C.CDLL:
+    """Load build/libdsa.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise DSAError(rc, lib().dsa_last_error().decode())
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def group_path(name: str) -> str:
+    """Path of a packaged group table (e.g. 'mys1080-v4', 'S216')."""
+    return os.path.join(GROUP_DIR, name + ".gz")
+
+
+class Group:
+    """A loaded group table (load_group, group.c:13-81)."""
+
+    def __init__(self, path: str):
+        h = C.c_void_p()
+        _check(lib().dsa_group_load(path.encode(), C.byref(h)))
+        self._h = h
+        info = GroupInfo()
+        _check(lib().dsa_group_info_get(h, C.byref(info)))
+        self.P = info.P
+        self.id = info.id
+        self.npool = info.npool
+        self.min_retr = info.min_retr
+        self.nn_retr = info.nn_retr
+        self.path = path
+
+    def tables(self):
+        """(mult[P,P], inv[P], ReTr[P], ImTr[P], pool[npool]) as numpy arrays."""
+        P = self.P
+        mult = np.empty((P, P), np.int32)
+        inv = np.empty(P, np.int32)
+        retr = np.empty(P, np.float64)
+        imtr = np.empty(P, np.float64)
+        pool = np.empty(self.npool, np.int32)
+        _check(lib().dsa_group_tables(self._h, _ptr(mult), _ptr(inv), _ptr(retr), _ptr(imtr), _ptr(pool)))
+        return mult, inv, retr, imtr, pool
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().dsa_group_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_group(path: str) -> Group:
+    return Group(path)
+
+
+def schedule(D: int, Nt: int, Nx: int, order: str = "colour"):
+    """[(mu, sites ndarray), ...]: the passes of one sweep, in order (host only)."""
+    n, m = C.c_uint32(), C.c_uint32()
+    o = DSA_ORDER[order]
+    _check(lib().dsa_schedule_query(D, Nt, Nx, o, C.byref(n), C.byref(m), None, None, None))
+    mu = np.empty(n.value, np.uint32)
+    cnt = np.empty(n.value, np.uint32)
+    sites = np.empty(m.value, np.uint32)
+    _check(lib().dsa_schedule_query(D, Nt, Nx, o, C.byref(n), C.byref(m), _ptr(mu), _ptr(cnt), _ptr(sites)))
+    out, off = [], 0
+    for q in range(n.value):
+        out.append((int(mu[q]), sites[off: off + cnt[q]]))
+        off += int(cnt[q])
+    return out
+
+
+class Chain:
+    """One Markov chain on the device (dym-mod-metro.cpp:222-268 set-up)."""
+
+    def __init__(self, group: Group, D: int, Nt: int, Nx: int, beta0: float, beta1: float, beta2: float,
+                 seed: int, start: str = "hot", rng: str = "philox", order: str = "colour"):
+        p = ChainParams(D, Nt, Nx, beta0, beta1, beta2, seed, DSA_START[start], DSA_RNG[rng], DSA_ORDER[order])
+        h = C.c_void_p()
+        _check(lib().dsa_chain_create(group._h, C.byref(p), C.byref(h)))
+        self._h = h
+        self.group = group
+        self.D, self.Nt, self.Nx = D, Nt, Nx
+        self.V = Nt * Nx ** (D - 1)
+
+    # -- dym-mod-metro.cpp:146-197
+    def update(self, n: int = 1) -> None:
+        _check(lib().dsa_chain_sweep(self._h, n))
+
+    sweep = update
+
+    def counters(self):
+        hit, acc = C.c_uint64(), C.c_uint64()
+        _check(lib().dsa_chain_counters(self._h, C.byref(hit), C.byref(acc)))
+        return hit.value, acc.value
+
+    @property
+    def sweeps_done(self) -> int:
+        s = C.c_uint64()
+        _check(lib().dsa_chain_get_sweep(self._h, C.byref(s)))
+        return s.value
+
+    @sweeps_done.setter
+    def sweeps_done(self, v: int) -> None:
+        _check(lib().dsa_chain_set_sweep(self._h, v))
+
+    def last_sweep_ms(self) -> float:
+        ms = C.c_double()
+        _check(lib().dsa_chain_last_sweep_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    # -- dym-mod-metro.cpp:281-318
+    def measure(self, wk: Optional[int] = None, wl: Optional[int] = None, sum: str = "tree") -> np.ndarray:
+        wk = self.Nx if wk is None else wk
+        wl = self.Nt if wl is None else wl
+        out = np.empty(4 + wk * wl, np.float64)
+        _check(lib().dsa_chain_measure(self._h, wk, wl, DSA_SUM[sum], _ptr(out)))
+        return out
+
+    # -- the gauge field a[i*D+mu] (lattice.h:9)
+    @property
+    def a(self) -> np.ndarray:
+        out = np.empty(self.V * self.D, np.int32)
+        _check(lib().dsa_chain_get_links(self._h, _ptr(out)))
+        return out
+
+    @a.setter
+    def a(self, value) -> None:
+        arr = np.ascontiguousarray(value, dtype=np.int32).reshape(-1)
+        if arr.size != self.V * self.D:
+            raise ValueError(f"expected {self.V * self.D} links, got {arr.size}")
+        _check(lib().dsa_chain_set_links(self._h, _ptr(arr)))
+
+    def rng_state(self) -> np.ndarray:
+        out = np.empty(self.V, np.uint32)
+        _check(lib().dsa_chain_get_rng_state(self._h, _ptr(out)))
+        return out
+
+    def set_rng_state(self, st) -> None:
+        arr = np.ascontiguousarray(st, dtype=np.uint32)
+        _check(lib().dsa_chain_set_rng_state(self._h, _ptr(arr)))
+
+    # -- get_staples / get_rect_loops (dym-mod-metro.cpp:63-142)
+    def staples(self):
+        ns, nr = 2 * (self.D - 1), 6 * (self.D - 1)
+        st = np.empty((self.V * self.D, ns), np.int32)
+        re = np.empty((self.V * self.D, nr), np.int32)
+        _check(lib().dsa_chain_staples(self._h, _ptr(st), _ptr(re)))
+        return st, re
+
+    # -- lattice.c:76-124 loop elements
+    def loops(self, kind: str, k: int = 0, l: int = 0) -> np.ndarray:
+        D, V = self.D, self.V
+        n = {"plaq": V * D * (D - 1) // 2, "rect": V * D * (D - 1), "poly": V // self.Nt,
+             "wilson": V * (D - 1)}[kind]
+        out = np.empty(n, np.int32)
+        _check(lib().dsa_chain_loops(self._h, LOOP_KIND[kind], k, l, _ptr(out)))
+        return out
+
+    def schedule_info(self):
+        n = C.c_uint32()
+        cols = (C.c_uint32 * 8)()
+        _check(lib().dsa_chain_schedule_info(self._h, C.byref(n), C.cast(cols, C.c_void_p)))
+        return n.value, list(cols)[: self.D]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().dsa_chain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

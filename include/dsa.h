@@ -1,0 +1,121 @@
+/*
+ * dsa.h -- C ABI of libdsa.so, the MI355X (gfx950) discrete-group Symanzik
+ * Monte Carlo.
+ *
+ * The reference exposes its path as global mutable state behind extern "C"
+ * headers (group.h:8-16, lattice.h:7-16) driven by dym-mod-metro's main; this
+ * ABI replaces that with opaque handles.  Every call returns 0 on success and
+ * a negative DSA_E* code on failure (never aborts); dsa_last_error() gives the
+ * calling thread's last message.  The caller owns handles and host buffers;
+ * the library owns device memory.  Calls on one handle must be serialised by
+ * the caller; distinct handles may be used from different threads.  A group
+ * must outlive every chain created from it.
+ *
+ * Link arrays crossing this ABI are int32 in the reference order a[i*D + mu]
+ * with site i = x0 + Nt*(x1 + Nx*(x2 + ...)) (lattice.c:39-51, x0 = time).
+ */
+#ifndef DSA_H
+#define DSA_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DSA_OK = 0,
+  DSA_E_ARG = -1,     /* invalid argument */
+  DSA_E_IO = -2,      /* cannot read a file */
+  DSA_E_FORMAT = -3,  /* malformed group table */
+  DSA_E_DEVICE = -4,  /* HIP runtime error or no device */
+  DSA_E_RANGE = -5,   /* size outside what the kernels support */
+  DSA_E_STATE = -6    /* call not valid in the handle's state */
+};
+
+enum { DSA_RNG_PHILOX = 0, DSA_RNG_MINSTD = 1 };
+enum { DSA_ORDER_COLOUR = 0, DSA_ORDER_REFERENCE = 1 };
+enum { DSA_START_HOT = 0, DSA_START_COLD = 1 };
+enum { DSA_SUM_TREE = 0, DSA_SUM_ORDERED = 1 };
+enum { DSA_LOOP_PLAQ = 0, DSA_LOOP_RECT = 1, DSA_LOOP_POLY = 2, DSA_LOOP_WILSON = 3 };
+
+typedef struct dsa_group dsa_group;
+typedef struct dsa_chain dsa_chain;
+
+typedef struct {
+  uint32_t P;         /* group order (group.h:9) */
+  int32_t id;         /* identity element (group.h:15) */
+  uint32_t npool;     /* proposal pool size, "small group size" (dym-mod-metro.cpp:245) */
+  double min_retr;    /* dym-mod-metro.cpp:239-241 */
+  double nn_retr;     /* dym-mod-metro.cpp:240-242 */
+} dsa_group_info;
+
+typedef struct {
+  uint32_t D, Nt, Nx;          /* lattice.h:7; V = Nt*Nx^(D-1) */
+  double beta0, beta1, beta2;  /* lattice.h:8 */
+  int32_t seed;                /* argv[8] of dym-mod-metro */
+  int32_t start;               /* DSA_START_HOT | DSA_START_COLD (dym-mod-metro.cpp:250-255) */
+  int32_t rng;                 /* DSA_RNG_PHILOX (default) | DSA_RNG_MINSTD (the reference's streams) */
+  int32_t order;               /* DSA_ORDER_COLOUR (default) | DSA_ORDER_REFERENCE (one-thread order) */
+} dsa_chain_params;
+
+/* Replaces load_group (group.c:13-81) and the proposal pool
+ * (dym-mod-metro.cpp:237-245).  Reads text or gzip. */
+int dsa_group_load(const char *path, dsa_group **out);
+/* Replaces reads of the globals P, id (group.h:9,15) and the pool statistics. */
+int dsa_group_info_get(const dsa_group *g, dsa_group_info *info);
+/* Copies the tables behind the globals ReTr, ImTr, mult, inv (group.h:10-14);
+ * any pointer may be NULL.  mult is P*P int32 row-major, pool is npool int32. */
+int dsa_group_tables(const dsa_group *g, int32_t *mult, int32_t *inv, double *retr, double *imtr, int32_t *pool);
+void dsa_group_free(dsa_group *g);
+
+/* Replaces main's set-up (dym-mod-metro.cpp:222-268): volume, per-site
+ * streams, hot or cold start, neighbour geometry, visiting order. */
+int dsa_chain_create(const dsa_group *g, const dsa_chain_params *p, dsa_chain **out);
+void dsa_chain_destroy(dsa_chain *c);
+/* n calls of update() (dym-mod-metro.cpp:146-197). */
+int dsa_chain_sweep(dsa_chain *c, uint32_t n);
+/* Cumulative Metropolis counters hit, acc (dym-mod-metro.cpp:60,194-195). */
+int dsa_chain_counters(dsa_chain *c, uint64_t *hit, uint64_t *acc);
+/* Sweeps done so far (the Philox counter); with set, resume from a checkpoint. */
+int dsa_chain_get_sweep(dsa_chain *c, uint64_t *sweep);
+int dsa_chain_set_sweep(dsa_chain *c, uint64_t sweep);
+/* The measurement block (dym-mod-metro.cpp:282-318; getpoly lattice.c:98-113):
+ * out[0..1] = mean Re/Im Polyakov loop, out[2] = mean plaquette ReTr,
+ * out[3] = mean 2x1 rectangle ReTr, out[4 + k*wl + l] = mean (k+1)x(l+1)
+ * spatial-temporal Wilson loop, k < wk <= Nx, l < wl <= Nt.  sum_mode
+ * DSA_SUM_TREE: fixed-shape FP64 tree (deterministic); DSA_SUM_ORDERED: the
+ * reference's one-thread summation order (one lane; small lattices). */
+int dsa_chain_measure(dsa_chain *c, uint32_t wk, uint32_t wl, int32_t sum_mode, double *out);
+/* Read / write the gauge field `a` (lattice.h:9, lattice.c:7,15). */
+int dsa_chain_get_links(dsa_chain *c, int32_t *a);
+int dsa_chain_set_links(dsa_chain *c, const int32_t *a);
+/* Minstd per-site stream states (V values) for checkpointing in that mode. */
+int dsa_chain_get_rng_state(dsa_chain *c, uint32_t *state);
+int dsa_chain_set_rng_state(dsa_chain *c, const uint32_t *state);
+/* get_staples / get_rect_loops (dym-mod-metro.cpp:63-142) of every link,
+ * computed by the sweep kernel's staple code: st[(i*D+mu)*2(D-1) + j],
+ * re[(i*D+mu)*6(D-1) + k]. */
+int dsa_chain_staples(dsa_chain *c, int32_t *st, int32_t *re);
+/* Element index of every closed loop of one kind (lattice.c:76-124), in the
+ * reference's enumeration: PLAQ [i][d1<d2]; RECT wilson(i,d1,d2,2,1) [i][d1!=d2];
+ * POLY polyakov of the t=0 sites ascending; WILSON wilson(i,d1,0,k+1,l+1) [i][d1>=1]. */
+int dsa_chain_loops(dsa_chain *c, int32_t kind, uint32_t k, uint32_t l, int32_t *out);
+/* Number of passes per sweep and colours per direction of the visiting order. */
+int dsa_chain_schedule_info(dsa_chain *c, uint32_t *npasses, uint32_t *ncolours);
+/* The visiting order a chain of this shape would use, computed on the host
+ * (no device needed).  Call with NULL arrays to get *npasses and *nsites;
+ * then pass_mu[npasses], pass_count[npasses], sites[nsites] receive, pass by
+ * pass, the direction and the ascending site list.  Replaces the reference's
+ * site_parity loop (dym-mod-metro.cpp:152-159, 261-268). */
+int dsa_schedule_query(uint32_t D, uint32_t Nt, uint32_t Nx, int32_t order, uint32_t *npasses, uint32_t *nsites,
+                       uint32_t *pass_mu, uint32_t *pass_count, uint32_t *sites);
+/* Milliseconds the last dsa_chain_sweep spent on the device (HIP events). */
+int dsa_chain_last_sweep_ms(dsa_chain *c, double *ms);
+
+const char *dsa_last_error(void);
+const char *dsa_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,203 @@
+"""ctypes wrapper of oracle/build/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: the checker for tests/, the smoke check and the
+CPU-baseline leg of bench.py.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+_lib = None
+
+RNG = {"minstd": 0, "philox": 1}
+ORDER = {"reference": 0, "colour": 1}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise ImportError(f"{LIB} missing: build the oracle (make -C oracle)")
+        L = C.CDLL(LIB)
+        vp, u32, i32, dbl, u64 = C.c_void_p, C.c_uint32, C.c_int32, C.c_double, C.c_uint64
+        sigs = {
+            "ogp_group_load": (vp, [C.c_char_p]),
+            "ogp_group_free": (None, [vp]),
+            "ogp_group_P": (u32, [vp]),
+            "ogp_group_id": (i32, [vp]),
+            "ogp_group_npool": (u32, [vp]),
+            "ogp_group_tables": (None, [vp, vp, vp, vp, vp, vp]),
+            "ogp_chain_new": (vp, [vp, u32, u32, u32, dbl, dbl, dbl, i32, i32, i32, i32]),
+            "ogp_chain_free": (None, [vp]),
+            "ogp_chain_sweep": (None, [vp, u32]),
+            "ogp_chain_get_links": (None, [vp, vp]),
+            "ogp_chain_set_links": (None, [vp, vp]),
+            "ogp_chain_counters": (None, [vp, C.POINTER(u64), C.POINTER(u64)]),
+            "ogp_chain_sweeps": (u64, [vp]),
+            "ogp_chain_set_sweeps": (None, [vp, u64]),
+            "ogp_chain_get_mstate": (None, [vp, vp]),
+            "ogp_chain_measure": (None, [vp, u32, u32, i32, vp]),
+            "ogp_chain_measure_abs": (None, [vp, u32, u32, vp]),
+            "ogp_chain_staples": (None, [vp, vp, vp]),
+            "ogp_chain_loops": (None, [vp, i32, u32, u32, vp]),
+            "ogp_chain_colouring": (C.c_long, [vp, vp, vp]),
+            "ogp_chain_readset": (u32, [vp, u32, u32, vp]),
+            "ogp_chain_S1": (dbl, [vp]),
+            "ogp_chain_sum_m1": (dbl, [vp]),
+            "ogp_philox": (None, [vp, vp, vp]),
+            "ogp_exp": (dbl, [dbl]),
+            "ogp_exp_many": (None, [vp, vp, C.c_size_t]),
+            "ogp_minstd_seed": (u64, [C.c_longlong]),
+            "ogp_canonical": (dbl, [C.POINTER(u64)]),
+            "ogp_randint": (C.c_int, [C.POINTER(u64), C.c_int, C.c_int]),
+        }
+        for n, (r, a) in sigs.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class OGroup:
+    def __init__(self, path: str):
+        self.h = lib().ogp_group_load(path.encode())
+        if not self.h:
+            raise ValueError(f"oracle: cannot load {path}")
+        self.P = lib().ogp_group_P(self.h)
+        self.id = lib().ogp_group_id(self.h)
+        self.npool = lib().ogp_group_npool(self.h)
+
+    def tables(self):
+        P = self.P
+        mult = np.empty((P, P), np.int32)
+        inv = np.empty(P, np.int32)
+        retr = np.empty(P, np.float64)
+        imtr = np.empty(P, np.float64)
+        pool = np.empty(self.npool, np.int32)
+        lib().ogp_group_tables(self.h, _p(mult), _p(inv), _p(retr), _p(imtr), _p(pool))
+        return mult, inv, retr, imtr, pool
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ogp_group_free(self.h)
+            self.h = None
+
+
+class OChain:
+    def __init__(self, g: OGroup, D, Nt, Nx, b0, b1, b2, seed, start="hot", rng="philox", order="colour"):
+        self.g = g
+        self.h = lib().ogp_chain_new(g.h, D, Nt, Nx, b0, b1, b2, seed, 1 if start == "cold" else 0,
+                                     RNG[rng], ORDER[order])
+        if not self.h:
+            raise ValueError("oracle: chain init failed")
+        self.D, self.Nt, self.Nx = D, Nt, Nx
+        self.V = Nt * Nx ** (D - 1)
+
+    def sweep(self, n=1):
+        lib().ogp_chain_sweep(self.h, n)
+
+    @property
+    def a(self):
+        out = np.empty(self.V * self.D, np.int32)
+        lib().ogp_chain_get_links(self.h, _p(out))
+        return out
+
+    @a.setter
+    def a(self, v):
+        arr = np.ascontiguousarray(v, dtype=np.int32).reshape(-1)
+        assert arr.size == self.V * self.D
+        lib().ogp_chain_set_links(self.h, _p(arr))
+
+    def counters(self):
+        h, a = C.c_uint64(), C.c_uint64()
+        lib().ogp_chain_counters(self.h, C.byref(h), C.byref(a))
+        return h.value, a.value
+
+    @property
+    def sweeps_done(self):
+        return lib().ogp_chain_sweeps(self.h)
+
+    @sweeps_done.setter
+    def sweeps_done(self, v):
+        lib().ogp_chain_set_sweeps(self.h, v)
+
+    def mstate(self):
+        out = np.empty(self.V, np.uint32)
+        lib().ogp_chain_get_mstate(self.h, _p(out))
+        return out
+
+    def measure(self, wk=None, wl=None, mode=0):
+        wk = self.Nx if wk is None else wk
+        wl = self.Nt if wl is None else wl
+        out = np.empty(4 + wk * wl, np.float64)
+        lib().ogp_chain_measure(self.h, wk, wl, mode, _p(out))
+        return out
+
+    def measure_abs(self, wk=None, wl=None):
+        wk = self.Nx if wk is None else wk
+        wl = self.Nt if wl is None else wl
+        out = np.empty(4 + wk * wl, np.float64)
+        lib().ogp_chain_measure_abs(self.h, wk, wl, _p(out))
+        return out
+
+    def staples(self):
+        ns, nr = 2 * (self.D - 1), 6 * (self.D - 1)
+        st = np.empty((self.V * self.D, ns), np.int32)
+        re = np.empty((self.V * self.D, nr), np.int32)
+        lib().ogp_chain_staples(self.h, _p(st), _p(re))
+        return st, re
+
+    def loops(self, kind, k=0, l=0):
+        D, V = self.D, self.V
+        n = {"plaq": V * D * (D - 1) // 2, "rect": V * D * (D - 1), "poly": V // self.Nt,
+             "wilson": V * (D - 1)}[kind]
+        out = np.empty(n, np.int32)
+        lib().ogp_chain_loops(self.h, {"plaq": 0, "rect": 1, "poly": 2, "wilson": 3}[kind], k, l, _p(out))
+        return out
+
+    def colouring(self):
+        col = np.empty(self.D * self.V, np.uint8)
+        ncol = np.zeros(8, np.uint32)
+        conflicts = lib().ogp_chain_colouring(self.h, _p(col), _p(ncol))
+        return col.reshape(self.D, self.V), ncol[: self.D], conflicts
+
+    def readset(self, i, d):
+        out = np.empty(512, np.uint32)
+        n = lib().ogp_chain_readset(self.h, i, d, _p(out))
+        return out[:n].copy()
+
+    def S1(self):
+        return lib().ogp_chain_S1(self.h)
+
+    def sum_m1(self):
+        return lib().ogp_chain_sum_m1(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ogp_chain_free(self.h)
+            self.h = None
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    o = np.empty(4, np.uint32)
+    lib().ogp_philox(_p(c), _p(k), _p(o))
+    return o
+
+
+def exp_many(x):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    lib().ogp_exp_many(_p(x), _p(y), x.size)
+    return y

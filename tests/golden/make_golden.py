@@ -1,0 +1,121 @@
+"""Writes the golden fixtures of tests/golden/ from the COMPILED REFERENCE.
+
+Run in the build container (needs /root/reference and `make -f oracle/ref.mk`):
+    python tests/golden/make_golden.py
+It runs oracle/_ref/dym-mod-metro (the reference driver built from its own
+sources) and oracle/_ref/ref_harness (the reference's own get_staples,
+get_rect_loops, update, plaquette, polyakov, wilson, getpoly called directly)
+and stores their inputs and outputs as small .npz / .txt files.  Nothing here
+is run on the GPU box: the tests only read the stored files.
+"""
+import json
+import os
+import platform
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = os.environ.get("DSA_REFERENCE", "/root/reference")
+REFBIN = os.path.join(REPO, "oracle", "_ref")
+
+
+def run(cmd, **kw):
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    kw.setdefault("cwd", tempfile.gettempdir())
+    return subprocess.run(cmd, check=True, capture_output=True, text=True, env=env, **kw).stdout
+
+
+def gmes_lines(text):
+    return [l for l in text.splitlines() if l.startswith(("GMES:", "RMES:", "ACC:"))]
+
+
+def loops_fixture(name, group, D, Nt, Nx, seed):
+    P = int(open(os.path.join(REF, "groups", group)).read().split(None, 1)[0])
+    V = Nt * Nx ** (D - 1)
+    rng = np.random.default_rng(seed)
+    cfg = rng.integers(0, P, size=V * D, dtype=np.int32)
+    with tempfile.TemporaryDirectory() as td:
+        cf = os.path.join(td, "cfg.i32")
+        cfg.tofile(cf)
+        run([os.path.join(REFBIN, "ref_harness"), "loops", os.path.join(REF, "groups", group), str(D), str(Nt),
+             str(Nx), cf, td])
+        ld = lambda f, t: np.fromfile(os.path.join(td, f), dtype=t)
+        ns, nr = 2 * (D - 1), 6 * (D - 1)
+        np.savez_compressed(
+            os.path.join(HERE, f"loops_{name}.npz"),
+            group=group, D=D, Nt=Nt, Nx=Nx, cfg=cfg,
+            staples=ld("staples.i32", np.int32).reshape(V * D, ns),
+            rect=ld("rect.i32", np.int32).reshape(V * D, nr),
+            r1=ld("r1.f64", np.float64), m1=ld("m1.f64", np.float64),
+            plaq=ld("plaq.i32", np.int32), rect21=ld("rect21.i32", np.int32),
+            poly=ld("poly.i32", np.int32), wil=ld("wil.i32", np.int32),
+            means=ld("means.f64", np.float64),
+        )
+
+
+def traj_fixture(name, group, D, Nt, Nx, b0, b1, b2, seed, nsweeps, start="hot", every=1):
+    V = Nt * Nx ** (D - 1)
+    with tempfile.TemporaryDirectory() as td:
+        run([os.path.join(REFBIN, "ref_harness"), "traj", os.path.join(REF, "groups", group), str(D), str(Nt),
+             str(Nx), repr(b0), repr(b1), repr(b2), str(seed), str(nsweeps), start, td, str(every)])
+        steps = sorted(int(f[6:-4]) for f in os.listdir(td) if f.startswith("links_"))
+        links = np.stack([np.fromfile(os.path.join(td, f"links_{s}.i32"), dtype=np.int32) for s in steps])
+        counters = np.fromfile(os.path.join(td, "counters.u64"), dtype=np.uint64).reshape(-1, 3)
+    np.savez_compressed(os.path.join(HERE, f"traj_{name}.npz"), group=group, D=D, Nt=Nt, Nx=Nx,
+                        beta=np.array([b0, b1, b2]), seed=seed, start=start, steps=np.array(steps),
+                        links=links, counters=counters)
+    assert links.shape[1] == V * D
+
+
+def cli_fixture(name, args, threads=1):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    with tempfile.TemporaryDirectory() as td:
+        out = subprocess.run([os.path.join(REFBIN, "dym-mod-metro")] + args, check=True, capture_output=True,
+                             text=True, env=env, cwd=td).stdout
+    lines = [l for l in out.splitlines() if not l.startswith("TM_")]
+    with open(os.path.join(HERE, f"cli_{name}.txt"), "w") as f:
+        f.write("# " + " ".join(["dym-mod-metro"] + args) + f"  (OMP_NUM_THREADS={threads})\n")
+        f.write("\n".join(lines) + "\n")
+
+
+def main():
+    if not os.path.exists(os.path.join(REFBIN, "ref_harness")):
+        sys.exit("build the reference first: make -f oracle/ref.mk")
+    g = "mys1080-v4"
+    loops_fixture("s1080_2x2", g, 4, 2, 2, 11)
+    loops_fixture("s1080_4x4", g, 4, 4, 4, 12)
+    loops_fixture("s1080_6x4", g, 4, 6, 4, 13)
+    loops_fixture("s216_4x6", "S216", 4, 4, 6, 14)
+    loops_fixture("mybi_3x3", "myBI", 4, 3, 3, 15)
+    loops_fixture("s108_d3_6x4", "S108", 3, 6, 4, 16)
+    # trajectories of the reference update() at one thread
+    traj_fixture("c1", g, 4, 2, 2, 0.0, 1.5, 0.1, 1, 110, every=10)
+    traj_fixture("s1080_4x4_b2", g, 4, 4, 4, 0.0, 3.0, -0.25, 5, 6)
+    traj_fixture("s1080_4x4_wilson", g, 4, 4, 4, 0.0, 2.0, 0.0, 3, 6)
+    traj_fixture("s216_4x6", "S216", 4, 4, 6, 0.0, 2.5, -0.2, 21, 3)
+    traj_fixture("mybi_3x3", "myBI", 4, 3, 3, 0.0, 1.0, 0.5, 77, 4)
+    # full CLI runs (config 1; statistics runs)
+    cli_fixture("c1", [os.path.join(REF, "groups", g), "4", "2", "2", "0", "1.5", "0.1", "1", "10", "100", "100"])
+    cli_fixture("stats_c1", [os.path.join(REF, "groups", g), "4", "2", "2", "0", "1.5", "0.1", "7", "10", "2000",
+                             "100"])
+    cli_fixture("stats_4x4", [os.path.join(REF, "groups", g), "4", "4", "4", "0", "3.0", "-0.25", "9", "4", "500",
+                              "100"])
+    meta = {
+        "generator": "tests/golden/make_golden.py",
+        "reference_build": "oracle/ref.mk (g++ -O3 -fopenmp -std=c++11, as the reference Makefile)",
+        "gcc": subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0],
+        "glibc": platform.libc_ver()[1],
+        "libstdc++": "GCC 11 headers (/usr/include/c++/11)",
+        "threads": 1,
+    }
+    with open(os.path.join(HERE, "meta.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

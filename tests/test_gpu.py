@@ -1,0 +1,235 @@
+"""Parity of the HIP path (through the C ABI) with the reference and the oracle.
+
+* element indices of staples and loops: bit-exact against the reference's own
+  functions (tests/golden/loops_*.npz);
+* trajectories in the reference's streams and one-thread order: bit-exact
+  against the reference update() (tests/golden/traj_*.npz) and the reference
+  binary's GMES output (cli_c1.txt);
+* Philox streams in the race-free colour order: bit-exact against the oracle;
+* FP64 observable sums: <= 1e-12 * sum|terms| against the reference's order;
+* statistics of the default mode: within 2 sigma (Gamma-method errors) of the
+  reference binary's runs (cli_stats_*.txt);
+* at the 32^3 x 64 benchmark size: size-independent properties.
+"""
+import glob
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO, group_file
+
+pytestmark = pytest.mark.gpu
+
+CLI = os.path.join(REPO, "dym_symanzik_action_amd", "build", "dym-mod-metro")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def fixtures(prefix):
+    return sorted(glob.glob(os.path.join(GOLDEN, prefix + "*.npz")))
+
+
+def gmes_block(text):
+    return [l for l in text.splitlines() if l.startswith(("GMES:", "RMES:", "ACC:"))]
+
+
+@pytest.mark.parametrize("path", fixtures("loops_"), ids=os.path.basename)
+def test_staples_and_loops_match_reference(dsa, dgroups, path):
+    f = np.load(path)
+    D, Nt, Nx = int(f["D"]), int(f["Nt"]), int(f["Nx"])
+    say("case", os.path.basename(path), D, Nt, Nx)
+    with dsa.Chain(dgroups(str(f["group"])), D, Nt, Nx, 0, 0, 0, 1, start="cold") as c:
+        c.a = f["cfg"]
+        assert np.array_equal(c.a, f["cfg"])
+        st, re = c.staples()
+        assert np.array_equal(st, f["staples"])
+        assert np.array_equal(re, f["rect"])
+        assert np.array_equal(c.loops("plaq"), f["plaq"])
+        assert np.array_equal(c.loops("rect"), f["rect21"])
+        assert np.array_equal(c.loops("poly"), f["poly"])
+        wil = [c.loops("wilson", k, l) for k in range(Nx) for l in range(Nt)]
+        assert np.array_equal(np.concatenate(wil), f["wil"])
+        # measurement block: the reference's order bit-exact, the tree to 1e-12 * sum|terms|
+        m_ord = c.measure(Nx, Nt, sum="ordered")
+        assert np.array_equal(m_ord, f["means"])
+        m_tree = c.measure(Nx, Nt, sum="tree")
+        mult, inv, retr, imtr, pool = dgroups(str(f["group"])).tables()
+        scale = np.empty_like(m_tree)
+        scale[0] = np.abs(retr[f["poly"]]).mean()
+        scale[1] = np.abs(imtr[f["poly"]]).mean()
+        scale[2] = np.abs(retr[f["plaq"]]).mean()
+        scale[3] = np.abs(retr[f["rect21"]]).mean()
+        scale[4:] = np.abs(retr[f["wil"]]).reshape(Nx * Nt, -1).mean(1)
+        assert np.all(np.abs(m_tree - f["means"]) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("path", fixtures("traj_"), ids=os.path.basename)
+def test_replay_matches_reference_update(dsa, dgroups, path):
+    f = np.load(path)
+    D, Nt, Nx = int(f["D"]), int(f["Nt"]), int(f["Nx"])
+    b = f["beta"]
+    say("case", os.path.basename(path))
+    counters = {int(r[0]): (int(r[1]), int(r[2])) for r in f["counters"]}
+    with dsa.Chain(dgroups(str(f["group"])), D, Nt, Nx, b[0], b[1], b[2], int(f["seed"]), start=str(f["start"]),
+                   rng="minstd", order="reference") as c:
+        assert np.array_equal(c.a, f["links"][0]) if int(f["steps"][0]) == 0 else True
+        done = 0
+        for step, links in zip(f["steps"], f["links"]):
+            c.update(int(step) - done)
+            done = int(step)
+            assert np.array_equal(c.a, links), f"links differ after {step} sweeps"
+            if step in counters:
+                assert c.counters() == counters[step]
+
+
+def test_cli_replay_reproduces_reference_output():
+    say("case cli --replay config 1")
+    out = subprocess.run([CLI, group_file("mys1080-v4"), "4", "2", "2", "0", "1.5", "0.1", "1", "10", "100", "100",
+                          "--replay"], capture_output=True, text=True, check=True, timeout=600).stdout
+    want = gmes_block(open(os.path.join(GOLDEN, "cli_c1.txt")).read())
+    assert gmes_block(out) == want
+    # header lines in the reference's order and format
+    head = [l for l in out.splitlines() if not l.startswith(("TM_", "GMES", "RMES", "ACC"))]
+    ref_head = [l for l in open(os.path.join(GOLDEN, "cli_c1.txt")).read().splitlines()[1:]
+                if not l.startswith(("GMES", "RMES", "ACC"))]
+    assert head[1:] == ref_head[1:]
+    assert head[0].split()[2:] == ref_head[0].split()[2:]
+
+
+@pytest.mark.parametrize("case", [
+    ("mys1080-v4", 4, 2, 2, 0.0, 1.5, 0.1, 1, "hot", 6),
+    ("mys1080-v4", 4, 4, 4, 0.0, 3.0, -0.25, 7, "hot", 4),
+    ("mys1080-v4", 4, 8, 4, 0.1, 2.0, -0.2, 8, "cold", 3),
+    ("mys1080-v4", 4, 6, 6, 0.0, 2.5, -0.1, 9, "hot", 2),
+    ("S216", 4, 4, 8, 0.0, 2.0, -0.25, 10, "hot", 3),
+    ("myBI", 4, 3, 3, 0.0, 1.0, 0.5, 11, "hot", 3),
+    ("S108", 3, 6, 4, 0.0, 1.5, 0.2, 12, "hot", 4),
+    ("myBT", 2, 8, 6, 0.0, 2.0, 0.1, 13, "hot", 5),
+    ("mys1080-v4", 4, 5, 4, 0.0, 2.0, -0.1, 14, "hot", 2),
+])
+@pytest.mark.parametrize("rng", ["philox", "minstd"])
+def test_colour_order_matches_oracle(dsa, dgroups, oracle, ogroups, case, rng):
+    name, D, Nt, Nx, b0, b1, b2, seed, start, nsw = case
+    say("case", name, D, Nt, Nx, rng)
+    o = oracle.OChain(ogroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour")
+    with dsa.Chain(dgroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour") as c:
+        assert np.array_equal(c.a, o.a), "start differs"
+        for s in range(nsw):
+            c.update(1)
+            o.sweep(1)
+            assert np.array_equal(c.a, o.a), f"links differ after sweep {s + 1}"
+        assert c.counters() == o.counters()
+        if rng == "minstd":
+            assert np.array_equal(c.rng_state(), o.mstate())
+        m = c.measure(min(Nx, 3), min(Nt, 3), sum="ordered")
+        assert np.array_equal(m, o.measure(min(Nx, 3), min(Nt, 3), mode=0))
+
+
+def test_config2_shape_one_sweep_matches_oracle(dsa, dgroups, oracle, ogroups):
+    """16^3 x 32 (524,288 links), S1080, hot seed 7, beta = (0, 3.0, -0.25)."""
+    say("case 16^3x32 one sweep vs oracle")
+    o = oracle.OChain(ogroups("mys1080-v4"), 4, 32, 16, 0.0, 3.0, -0.25, 7, rng="philox", order="colour")
+    with dsa.Chain(dgroups("mys1080-v4"), 4, 32, 16, 0.0, 3.0, -0.25, 7) as c:
+        assert np.array_equal(c.a, o.a)
+        c.update(1)
+        o.sweep(1)
+        assert np.array_equal(c.a, o.a)
+        assert c.counters() == o.counters()
+        m = c.measure(8, 8)
+        want = o.measure(8, 8, mode=0)
+        scale = o.measure_abs(8, 8)
+        assert np.all(np.abs(m - want) <= 1e-12 * scale)
+
+
+def test_resume_from_checkpoint_is_bit_exact(dsa, dgroups):
+    say("case checkpoint/resume")
+    g = dgroups("mys1080-v4")
+    with dsa.Chain(g, 4, 8, 8, 0, 3.0, -0.25, 5) as a, dsa.Chain(g, 4, 8, 8, 0, 3.0, -0.25, 5) as b:
+        a.update(4)
+        b.update(2)
+        links, sw = b.a, b.sweeps_done
+    with dsa.Chain(g, 4, 8, 8, 0, 3.0, -0.25, 5, start="cold") as r:
+        r.a = links
+        r.sweeps_done = sw
+        r.update(2)
+        with dsa.Chain(g, 4, 8, 8, 0, 3.0, -0.25, 5) as a2:
+            a2.update(4)
+            assert np.array_equal(r.a, a2.a)
+
+
+def _stats_compare(ref_lines, got_lines, nx, nt, keys):
+    from stats_util import gamma_method, parse_gmes
+
+    ref = parse_gmes(ref_lines, nx, nt)
+    got = parse_gmes(got_lines, nx, nt)
+    report = []
+    for k in keys:
+        m1, e1, _ = gamma_method(ref[k])
+        m2, e2, _ = gamma_method(got[k])
+        z = abs(m1 - m2) / max(np.hypot(e1, e2), 1e-300)
+        report.append((k, m1, e1, m2, e2, z))
+    return report
+
+
+@pytest.mark.parametrize("fixture,args,nx,nt", [
+    ("cli_stats_c1.txt", ["4", "2", "2", "0", "1.5", "0.1", "71", "10", "2000", "100"], 2, 2),
+    ("cli_stats_4x4.txt", ["4", "4", "4", "0", "3.0", "-0.25", "93", "4", "500", "100"], 4, 4),
+])
+def test_statistics_agree_with_reference(fixture, args, nx, nt):
+    say("case stats", fixture)
+    out = subprocess.run([CLI, group_file("mys1080-v4")] + args, capture_output=True, text=True, check=True,
+                         timeout=900).stdout
+    ref = open(os.path.join(GOLDEN, fixture)).read().splitlines()
+    keys = ["plaq", "rect", "absP", "W1x1", f"W{nx}x{nt}", "W1x2", "W2x1"]
+    rep = _stats_compare(ref, out.splitlines(), nx, nt, keys)
+    for k, m1, e1, m2, e2, z in rep:
+        say(f"  {k}: ref {m1:.6f}+-{e1:.6f}  gpu {m2:.6f}+-{e2:.6f}  z={z:.2f}")
+    assert all(z < 2.0 for *_, z in rep), rep
+
+
+def test_benchmark_size_properties(dsa, dgroups):
+    """32^3 x 64 (8,388,608 links): cold start stays at the identity's ReTr on
+    every loop before sweeping; two chains with one seed agree bit for bit;
+    every link is hit NHIT times per sweep; the tree plaquette mean equals a
+    numpy restatement of the same loops to 1e-12."""
+    say("case 32^3x64 properties")
+    g = dgroups("mys1080-v4")
+    mult, inv, retr, *_ = g.tables()
+    with dsa.Chain(g, 4, 64, 32, 0, 3.0, -0.25, 11, start="cold") as c:
+        m = c.measure(0, 0)
+        assert np.all(m[[0, 2, 3]] == -3.0)
+        c.update(2)
+        hit, acc = c.counters()
+        assert hit == 2 * 20 * c.V * 4 and 0 < acc < hit
+        a1 = c.a
+    with dsa.Chain(g, 4, 64, 32, 0, 3.0, -0.25, 11, start="cold") as c2:
+        c2.update(2)
+        assert np.array_equal(c2.a, a1)
+        m = c2.measure(0, 0)
+    # numpy plaquette mean over the same configuration
+    D, Nt, Nx = 4, 64, 32
+    V = Nt * Nx ** 3
+    a = a1.reshape(V, D)
+    idx = np.arange(V)
+    step = [1, Nt, Nt * Nx, Nt * Nx * Nx]
+    ext = [Nt, Nx, Nx, Nx]
+
+    def fwd(i, d):
+        x = (i // step[d]) % ext[d]
+        return i + ((x + 1) % ext[d] - x) * step[d]
+
+    tot = 0.0
+    for d1 in range(D):
+        for d2 in range(d1 + 1, D):
+            p = mult[a[:, d1], a[fwd(idx, d1), d2]]
+            p = mult[p, inv[a[fwd(idx, d2), d1]]]
+            p = mult[p, inv[a[:, d2]]]
+            tot += retr[p].sum()
+    want = tot / (V * 6)
+    assert abs(m[2] - want) <= 1e-12 * 3.0

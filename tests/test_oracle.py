@@ -1,0 +1,319 @@
+"""The oracle (oracle/, a C restatement of the reference path) pinned against
+the compiled reference's outputs (tests/golden/, written by make_golden.py from
+the reference's own functions) and against facts the mathematics fixes."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, group_file
+
+
+def fixtures(prefix):
+    return sorted(glob.glob(os.path.join(GOLDEN, prefix + "*.npz")))
+
+
+def gmes_block(path):
+    return [l for l in open(path).read().splitlines() if l.startswith(("GMES:", "RMES:", "ACC:"))]
+
+
+# ---------------------------------------------------------------- streams
+
+def test_philox_known_answers(oracle):
+    # Random123 kat_vectors for philox4x32_10
+    cases = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in cases:
+        assert list(oracle.philox(ctr, key)) == want
+
+
+def test_minstd_standard_known_answer(oracle):
+    # [rand.predef]: the 10000th output of a default-constructed minstd_rand0 is 1043618065
+    import ctypes
+
+    x = ctypes.c_uint64(1)
+    lib = oracle.lib()
+    for _ in range(10000):
+        v = x.value * 16807 % 2147483647
+        x.value = v
+    assert x.value == 1043618065
+    # the oracle's seeding rule: seed(s) = s mod m, 0 -> 1; negatives wrap modulo 2^64
+    assert lib.ogp_minstd_seed(1) == 1
+    assert lib.ogp_minstd_seed(0) == 1
+    assert lib.ogp_minstd_seed(2147483647) == 1
+    for s in (-4, -5, -123456):
+        want = (2**64 + s) % 2147483647
+        assert lib.ogp_minstd_seed(s) == (want if want else 1)
+
+
+def test_canonical_matches_two_draw_formula(oracle):
+    import ctypes
+
+    lib = oracle.lib()
+    x = ctypes.c_uint64(12345)
+    y = 12345
+    R = 2147483646
+    for _ in range(2000):
+        u = lib.ogp_canonical(ctypes.byref(x))
+        y1 = y * 16807 % 2147483647
+        y2 = y1 * 16807 % 2147483647
+        y = y2
+        # sum_{k} (x_k - 1) R^k / R^2 evaluated in doubles as libstdc++ does
+        s = float(y1 - 1) + float(y2 - 1) * float(R)
+        want = s / float(R * R)
+        assert u == want and 0.0 <= u < 1.0
+
+
+def test_exp_against_libm(oracle):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-745, 709, 200000), rng.uniform(-50, 0, 200000), rng.uniform(-1, 1, 100000),
+                        [0.0, -0.0, 1e-300, -1e-300, 709.7, -745.1, -744.0, -708.5, 1.0, -1.0, np.log(2)]])
+    y = oracle.exp_many(x)
+    ref = np.exp(x)
+    ok = np.isfinite(ref) & (ref > 2.3e-308)
+    ulp = np.abs(y[ok].view(np.int64) - ref[ok].view(np.int64))
+    # within one unit in the last place of glibc's exp everywhere; equal in ~89 %
+    # of cases.  An accept decision exp(x) > u can differ from glibc's only when
+    # u falls inside that one-ulp gap (probability < 2.3e-16 per hit).
+    assert ulp.max() <= 1, ulp.max()
+    assert (ulp == 0).mean() > 0.85
+    # subnormal range: relative agreement
+    sub = (ref > 0) & (ref <= 2.3e-308)
+    assert np.all(np.abs(y[sub] - ref[sub]) <= 2 * 4.95e-324 + 1e-15 * ref[sub])
+    assert oracle.lib().ogp_exp(0.0) == 1.0
+    assert oracle.lib().ogp_exp(-800.0) == 0.0
+    assert np.isinf(oracle.lib().ogp_exp(710.0))
+
+
+# ---------------------------------------------------------------- groups
+
+@pytest.mark.parametrize("name,P,gid,npool", [("mys1080-v4", 1080, 0, 72), ("S216", 216, 10, 54),
+                                              ("S108", 108, None, 18), ("myBT", 24, None, None),
+                                              ("Q8", 8, None, None)])
+def test_group_tables(ogroups, name, P, gid, npool):
+    g = ogroups(name)
+    assert g.P == P
+    if gid is not None:
+        assert g.id == gid
+    if npool is not None:
+        assert g.npool == npool
+    mult, inv, retr, imtr, pool = g.tables()
+    e = np.arange(P)
+    # identity row/column, inverses, Latin square
+    assert np.array_equal(mult[g.id], e) and np.array_equal(mult[:, g.id], e)
+    assert np.all(mult[e, inv] == g.id) and np.all(mult[inv, e] == g.id)
+    assert all(len(set(r)) == P for r in mult[: min(P, 64)])
+    # associativity: exhaustive for small groups, 2e5 random triples otherwise
+    if P ** 3 <= 2_000_000:
+        a, b, c = np.meshgrid(e, e, e, indexing="ij")
+        assert np.array_equal(mult[mult[a, b], c], mult[a, mult[b, c]])
+    else:
+        rng = np.random.default_rng(7)
+        a, b, c = rng.integers(0, P, (3, 200000))
+        assert np.array_equal(mult[mult[a, b], c], mult[a, mult[b, c]])
+    # identity has the smallest ReTr (-Tr of the identity = -N)
+    assert retr[g.id] == retr.min()
+    # pool = elements at the second-smallest ReTr (dym-mod-metro.cpp:239-243)
+    second = np.min(retr[retr > retr.min()])
+    assert np.array_equal(pool, np.nonzero((retr > retr.min()) & (retr < second + 1e-6))[0])
+    # pool closed under inversion
+    assert set(inv[pool]) == set(pool)
+
+
+def test_group_bad_files(oracle, tmp_path):
+    bad = tmp_path / "bad"
+    bad.write_text("3\n0 0 0\n0 0 0\n0 1 2 1 2 0 2 0\n")  # truncated table
+    assert not oracle.lib().ogp_group_load(str(bad).encode())
+    noid = tmp_path / "noid"
+    noid.write_text("2\n0 0\n0 0\n1 1 1 1\n")
+    assert not oracle.lib().ogp_group_load(str(noid).encode())
+    assert not oracle.lib().ogp_group_load(str(tmp_path / "missing").encode())
+
+
+# ----------------------------------------------------- against the reference
+
+@pytest.mark.parametrize("path", fixtures("loops_"), ids=os.path.basename)
+def test_loops_and_staples_match_reference(oracle, ogroups, path):
+    f = np.load(path)
+    g = ogroups(str(f["group"]))
+    D, Nt, Nx = int(f["D"]), int(f["Nt"]), int(f["Nx"])
+    c = oracle.OChain(g, D, Nt, Nx, 0, 0, 0, 1, rng="minstd", order="reference")
+    c.a = f["cfg"]
+    st, re = c.staples()
+    assert np.array_equal(st, f["staples"])
+    assert np.array_equal(re, f["rect"])
+    mult, inv, retr, imtr, pool = g.tables()
+    b = f["cfg"]
+    r1 = np.zeros(len(b))
+    m1 = np.zeros(len(b))
+    for j in range(st.shape[1]):
+        r1 = r1 + retr[mult[b, st[:, j]]]
+    for k in range(re.shape[1]):
+        m1 = m1 + retr[mult[b, re[:, k]]]
+    assert np.array_equal(r1, f["r1"]) and np.array_equal(m1, f["m1"])
+    assert np.array_equal(c.loops("plaq"), f["plaq"])
+    assert np.array_equal(c.loops("rect"), f["rect21"])
+    assert np.array_equal(c.loops("poly"), f["poly"])
+    wil = []
+    for k in range(Nx):
+        for l in range(Nt):
+            wil.append(c.loops("wilson", k, l))
+    assert np.array_equal(np.concatenate(wil), f["wil"])
+    m = c.measure(Nx, Nt, mode=0)
+    assert np.array_equal(m, f["means"])  # same summation order: bit-identical
+    mh = c.measure(Nx, Nt, mode=1)
+    scale = c.measure_abs(Nx, Nt)
+    assert np.all(np.abs(mh - f["means"]) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("path", fixtures("traj_"), ids=os.path.basename)
+def test_trajectory_matches_reference_update(oracle, ogroups, path):
+    f = np.load(path)
+    g = ogroups(str(f["group"]))
+    D, Nt, Nx = int(f["D"]), int(f["Nt"]), int(f["Nx"])
+    b = f["beta"]
+    c = oracle.OChain(g, D, Nt, Nx, b[0], b[1], b[2], int(f["seed"]), start=str(f["start"]), rng="minstd",
+                      order="reference")
+    done = 0
+    counters = {int(r[0]): (int(r[1]), int(r[2])) for r in f["counters"]}
+    for step, links in zip(f["steps"], f["links"]):
+        c.sweep(int(step) - done)
+        done = int(step)
+        assert np.array_equal(c.a, links), f"links differ after {step} sweeps"
+        if step in counters:
+            assert c.counters() == counters[step]
+
+
+def test_cli_config1_matches_reference(tmp_path):
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "build", "oracle_metro")
+    out = subprocess.run([exe, group_file("mys1080-v4"), "4", "2", "2", "0", "1.5", "0.1", "1", "10", "100", "100"],
+                         capture_output=True, text=True, check=True).stdout
+    got = [l for l in out.splitlines() if l.startswith(("GMES:", "RMES:", "ACC:"))]
+    assert got == gmes_block(os.path.join(GOLDEN, "cli_c1.txt"))
+
+
+# ------------------------------------------------------------ colouring (F1)
+
+@pytest.mark.parametrize("shape", [(4, 2, 2), (4, 4, 4), (4, 8, 4), (4, 4, 8), (4, 6, 6), (4, 12, 12), (4, 3, 3),
+                                   (4, 5, 4), (3, 6, 4), (2, 8, 6), (4, 32, 16)])
+def test_colouring_is_race_free(oracle, ogroups, shape):
+    D, Nt, Nx = shape
+    c = oracle.OChain(ogroups("S108"), D, Nt, Nx, 0, 1, 0, 1, rng="philox", order="colour")
+    col, ncol, conflicts = c.colouring()
+    assert conflicts == 0
+    assert np.all(ncol >= 2)
+
+
+def test_reference_parity_colouring_races_with_rectangles(oracle, ogroups):
+    """The reference's two-colour checkerboard: links of one parity share
+    rectangles (so its parallel sweep races for beta2 != 0) but not plaquettes."""
+    D, Nt, Nx = 4, 8, 8
+    c = oracle.OChain(ogroups("S108"), D, Nt, Nx, 0, 1, 0, 1, rng="philox", order="colour")
+    V = c.V
+    pos = np.stack(np.unravel_index(np.arange(V), (Nx, Nx, Nx, Nt))[::-1])
+    par = pos.sum(0) & 1
+    rect_conf = plaq_conf = 0
+    for i in range(0, V, 7):
+        rs = c.readset(i, 0)
+        sites, dirs = rs // D, rs % D
+        same = (dirs == 0) & (sites != i) & (par[sites] == par[i])
+        # the first 3 reads of each plaquette staple block are plaquette links
+        rect_conf += int(same.sum())
+        plaq_sites = []
+        for q in range(D - 1):
+            plaq_sites += list(rs[6 * q: 6 * q + 6])
+        plaq_sites = np.array(plaq_sites)
+        plaq_conf += int(((plaq_sites % D == 0) & (plaq_sites // D != i) & (par[plaq_sites // D] == par[i])).sum())
+    assert rect_conf > 0
+    assert plaq_conf == 0
+
+
+# ------------------------------------------------------- mathematical pins
+
+def test_cold_start_loops_are_identity(oracle, ogroups):
+    g = ogroups("mys1080-v4")
+    c = oracle.OChain(g, 4, 4, 4, 0, 2.0, -0.2, 3, start="cold", rng="philox", order="colour")
+    m = c.measure(4, 4, mode=0)
+    retr_id = g.tables()[2][g.id]
+    assert retr_id == -3.0
+    assert np.all(m[[0, 2, 3]] == -3.0) and np.all(m[4:] == -3.0) and m[1] == 0.0
+
+
+def test_up_staple_closes_the_plaquette(oracle, ogroups):
+    g = ogroups("mys1080-v4")
+    mult, inv, *_ = g.tables()
+    D, Nt, Nx = 4, 4, 4
+    c = oracle.OChain(g, D, Nt, Nx, 0, 1, 0, 5, rng="philox", order="colour")
+    st, _ = c.staples()
+    a = c.a
+    plaq = c.loops("plaq").reshape(c.V, -1)
+    pairs = [(d1, d2) for d1 in range(D) for d2 in range(d1 + 1, D)]
+    for mu in range(D):
+        for j, nu in enumerate([n for n in range(D) if n != mu]):
+            if mu < nu:  # U_mu U_nu(x+mu) U_mu(x+nu)^-1 U_nu^-1 = b * up-staple
+                p = plaq[:, pairs.index((mu, nu))]
+                assert np.array_equal(mult[a[mu::D], st[mu::D, 2 * j]], p)
+
+
+def test_gauge_transformation_conjugates_loops(oracle, ogroups):
+    g = ogroups("S216")
+    mult, inv, *_ = g.tables()
+    D, Nt, Nx = 4, 4, 4
+    c = oracle.OChain(g, D, Nt, Nx, 0, 1, 0, 5, rng="philox", order="colour")
+    V = c.V
+    a = c.a.reshape(V, D)
+    plaq0 = c.loops("plaq").reshape(V, -1)
+    rng = np.random.default_rng(3)
+    gt = rng.integers(0, g.P, V)
+    step = [1, Nt, Nt * Nx, Nt * Nx * Nx]
+    ext = [Nt, Nx, Nx, Nx]
+    idx = np.arange(V)
+    new = np.empty_like(a)
+    for mu in range(D):
+        x = (idx // step[mu]) % ext[mu]
+        nb = idx + ((x + 1) % ext[mu] - x) * step[mu]
+        new[:, mu] = mult[mult[gt, a[:, mu]], inv[gt[nb]]]
+    c.a = new.reshape(-1)
+    plaq1 = c.loops("plaq").reshape(V, -1)
+    assert np.array_equal(plaq1, mult[mult[gt[:, None], plaq0], inv[gt][:, None]])
+
+
+def test_zero_coupling_accepts_everything(oracle, ogroups):
+    c = oracle.OChain(ogroups("mys1080-v4"), 4, 4, 4, 0, 0, 0, 9, rng="philox", order="colour")
+    c.sweep(2)
+    hit, acc = c.counters()
+    assert hit == acc == 2 * 20 * c.V * 4
+
+
+def test_single_link_heat_bath_distribution(oracle, ogroups):
+    """With every other link frozen, NHIT Metropolis hits on one link sample
+    exp(-beta1 sum_j ReTr(b s_j)); compare histograms over many sweeps of a
+    2^4 lattice restricted to direction 0 is not possible through the sweep,
+    so check detailed balance on the accept rule directly."""
+    g = ogroups("S108")
+    mult, inv, retr, imtr, pool = g.tables()
+    rng = np.random.default_rng(0)
+    staples = rng.integers(0, g.P, 6)
+    beta = 0.7
+    S = lambda b: beta * retr[mult[b, staples]].sum()
+    b = g.id
+    counts = np.zeros(g.P)
+    for t in range(60000):
+        bn = mult[b, pool[int(len(pool) * rng.random())]]
+        dS = S(bn) - S(b)
+        p = 1.0 if dS <= 0 else oracle.lib().ogp_exp(-dS)
+        if p > rng.random():
+            b = bn
+        counts[b] += 1
+    w = np.exp(-np.array([S(e) for e in range(g.P)]))
+    w /= w.sum()
+    # total-variation distance between the chain's histogram and the Boltzmann weights
+    tv = 0.5 * np.abs(counts / counts.sum() - w).sum()
+    assert tv < 0.06
