@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: Metropolis link-updates/s of one S(1080) chain at 32^3 x 64.
+
+A step is one full sweep: every one of the 8,388,608 links gets its 24
+staples built and NHIT = 20 Metropolis hits (dym-mod-metro.cpp:146-197), in 32
+race-free colour passes (one kernel launch each).  Workload: hot start seed
+11, beta = (0, 3.0, -0.25), Philox streams, S(1080) tables resident on the
+device before the timed region.
+
+Prints one JSON line.  Besides the contract fields it carries
+  roofline      the sweep kernel against HBM (66 algorithmic bytes of link
+                traffic per link-update), with per-launch times from HIP
+                events on the chain's stream, and its random-gather ceiling
+                (608 table gathers per link-update vs a measured probe);
+  cpu_baseline  the reference binary (oracle/_ref/dym-mod-metro, built from
+                the reference's sources) timed on this host's cores for one
+                sweep of the same workload, by its own TM_END: update line.
+"""
+import argparse
+import gzip
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+D, NT, NX = 4, 64, 32
+V = NT * NX ** 3
+LINKS = V * D
+BETA = (0.0, 3.0, -0.25)
+SEED = 11
+NHIT = 20
+BYTES_PER_LINK_UPDATE = 66      # 64 B of link reads + 2 B write per updated link
+GATHERS_PER_LINK_UPDATE = 608   # 84 staple + 24 initial + 20*25 hit-loop table gathers
+HBM_PEAK_GBS = 8000.0
+PMC_FILE = os.path.join(REPO, "profiles", "k_sweep_32x64_pmc.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary."""
+    try:
+        for line in open(PMC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(group_gz, cores):
+    """Two sweeps of the same workload by the reference binary, on `cores` threads."""
+    ref = os.path.join(REPO, "oracle", "_ref", "dym-mod-metro")
+    if not os.path.exists(ref):
+        return None
+    td = tempfile.mkdtemp(prefix="dsa_cpu_")
+    try:
+        gtxt = os.path.join(td, "mys1080-v4")
+        with gzip.open(group_gz, "rb") as fi, open(gtxt, "wb") as fo:
+            shutil.copyfileobj(fi, fo)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        nsw = 2
+        cmd = [ref, gtxt, str(D), str(NT), str(NX), *map(repr, BETA), str(SEED), str(nsw), "1", "0"]
+        p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, env=env, cwd=td)
+        t = None
+        deadline = time.time() + 300
+        for line in p.stdout:
+            if line.startswith("TM_END:") and "update" in line:
+                t = float(line.split("done in")[1].split()[0])
+                break
+            if time.time() > deadline:
+                break
+        p.kill()
+        p.wait()
+        if t is None:
+            return None
+        return {"value": nsw * LINKS / t, "unit": "link-updates/s", "cores": cores, "kind": "reference",
+                "sample": f"{nsw} sweeps of 32^3x64 S1080 hot seed {SEED} beta={BETA} ({nsw} x 8,388,608 "
+                          "link-updates), "
+                          f"reference dym-mod-metro built from its sources (-O3 -fopenmp), OMP_NUM_THREADS={cores}, "
+                          f"timed by its own TM_END: update line ({t:.2f} s)"}
+    finally:
+        shutil.rmtree(td, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run a plain copy kernel of this size (scale for the FETCH/WRITE_SIZE counters)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "independent chains shard across GPUs with no communication, see the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import dym_symanzik_action_amd as dsa
+
+    gpath = dsa.group_path("mys1080-v4")
+    g = dsa.load_group(gpath)
+    log("bench: creating the 32^3x64 chain")
+    c = dsa.Chain(g, D, NT, NX, *BETA, SEED, start="hot", rng="philox", order="colour")
+    npass, ncol = c.schedule_info()
+    log(f"bench: {npass} passes per sweep, colours per direction {ncol}; warmup {args.warmup}")
+    import ctypes as C
+
+    c.update(args.warmup)
+    hit0, acc0 = c.counters()
+    log(f"bench: timing {args.steps} sweeps")
+    # every pass launch of the timed sweeps is bracketed by HIP events on the
+    # chain's stream: the roofline's per-launch time comes from the same sweeps
+    dsa.lib().dsa_chain_set_profiling(c._h, 1)
+    t0 = time.perf_counter()
+    c.update(args.steps)          # returns after the stream has drained
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms, nl = C.c_double(), C.c_uint64()
+    dsa.lib().dsa_chain_kernel_time(c._h, C.byref(ms), C.byref(nl))
+    dsa.lib().dsa_chain_set_profiling(c._h, 0)
+    hit1, acc1 = c.counters()
+    assert hit1 - hit0 == args.steps * LINKS * NHIT
+    value = args.steps * LINKS / elapsed
+    avg_launch_s = ms.value * 1e-3 / nl.value
+    units = LINKS // npass
+    achieved = BYTES_PER_LINK_UPDATE * units / avg_launch_s / 1e9
+    traffic = read_traffic()
+    gather_rate = GATHERS_PER_LINK_UPDATE * units / avg_launch_s
+    gp = C.c_double()
+    best = 0.0
+    for blocks in (2048, 4096, 8192):
+        if dsa.lib().dsa_probe_gather(g._h, blocks, 64, C.byref(gp)) == 0:
+            best = max(best, gp.value)
+    m = c.measure(0, 0)
+    c.close()
+    copy_ms = None
+    if args.copy_probe_mb:
+        cm = C.c_double()
+        if dsa.lib().dsa_probe_copy(args.copy_probe_mb << 20, 4, C.byref(cm)) == 0:
+            copy_ms = cm.value
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cores = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            cores = min(16, os.cpu_count() or 1)
+        log(f"bench: CPU baseline, reference binary on {cores} threads")
+        cpu = cpu_baseline(gpath, cores)
+
+    out = {
+        "metric": "Metropolis link-updates/s at 32^3x64 S1080",
+        "value": value,
+        "unit": "link-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16 group indices, f64 action sums",
+        "data": "synthetic: hot start from Philox seed 11 (no dataset involved)",
+        "config": {"workload": "single chain 32^3x64 (8,388,608 uint16 links), S(1080) table, hot seed 11, "
+                               "beta=(0,3.0,-0.25), NHIT=20, Philox4x32-10, race-free colour order",
+                   "lattice": "32^3x64", "group": "mys1080-v4", "links": LINKS, "passes_per_sweep": npass,
+                   "parallelism": "single GPU"},
+        "acceptance": (acc1 - acc0) / (hit1 - hit0),
+        "plaquette_after": m[2],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_sweep<4>", "avg_launch_ms": avg_launch_s * 1e3, "units_per_launch": units,
+                     "algorithmic_bytes_per_unit": BYTES_PER_LINK_UPDATE,
+                     "gather": {"achieved_per_s": gather_rate, "ceiling_per_s": best,
+                                "frac": gather_rate / best if best else None,
+                                "per_unit": GATHERS_PER_LINK_UPDATE}},
+        "cpu_baseline": cpu,
+    }
+    if copy_ms is not None:
+        out["copy_probe"] = {"bytes": args.copy_probe_mb << 20, "ms": copy_ms}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -9,6 +9,7 @@ package without the built library raises.
 """
 from .dsa import (  # noqa: F401
     GROUP_DIR,
+    Batch,
     Chain,
     DSAError,
     Group,
@@ -18,4 +19,4 @@ from .dsa import (  # noqa: F401
     schedule,
 )
 
-__all__ = ["Chain", "Group", "DSAError", "load_group", "group_path", "lib", "GROUP_DIR", "schedule"]
+__all__ = ["Batch", "Chain", "Group", "DSAError", "load_group", "group_path", "lib", "GROUP_DIR", "schedule"]

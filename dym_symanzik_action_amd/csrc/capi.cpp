@@ -12,36 +12,22 @@
 #include "../../include/dsa.h"
 #include "dsa_math.hpp"
 #include "group.hpp"
+#include "internal.hpp"
 #include "kernels.hpp"
+#include "probe.hpp"
 #include "schedule.hpp"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string &msg) {
+using dsa::DeviceTables;
+using dsa::fail;
+
+int dsa::fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
-
-#define HIP_TRY(expr)                                                                            \
-  do {                                                                                           \
-    hipError_t e_ = (expr);                                                                      \
-    if (e_ != hipSuccess) return fail(DSA_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-struct DeviceTables {
-  uint16_t *mult = nullptr, *inv = nullptr, *pool = nullptr;
-  double *retr = nullptr, *imtr = nullptr;
-};
-
-}  // namespace
-
-struct dsa_group {
-  dsa::Group g;
-  std::mutex m;
-  std::map<int, DeviceTables> dev;
-};
 
 struct dsa_chain {
   dsa_group *grp = nullptr;
@@ -69,11 +55,62 @@ struct dsa_chain {
   uint64_t sweep = 0, hit = 0;
   double last_ms = 0;
   int max_blocks = 2048;
+  // per-launch timing (measurement mode): one event pair per pass
+  bool profiling = false;
+  std::vector<hipEvent_t> pev;
+  double kern_ms = 0;
+  uint64_t kern_launches = 0;
 };
 
-namespace {
+dsa::Tables dsa::tables_view(const dsa_group *g, const DeviceTables &t) {
+  dsa::Tables T{};
+  T.mult = t.mult;
+  T.inv = t.inv;
+  T.retr = t.retr;
+  T.imtr = t.imtr;
+  T.pool = t.pool;
+  T.P = g->g.P;
+  T.npool = (uint32_t)g->g.pool.size();
+  T.id = (uint32_t)g->g.id;
+  return T;
+}
 
-int upload_tables(dsa_group *g, int device, DeviceTables &out) {
+int dsa::check_params(const dsa_chain_params *p) {
+  if (p->D < 2 || p->D > (uint32_t)dsa::kDMax)
+    return fail(DSA_E_RANGE, "D = " + std::to_string(p->D) + " outside the supported 2..5");
+  if (p->Nt < 1 || p->Nx < 1) return fail(DSA_E_ARG, "Nt and Nx must be positive");
+  uint64_t V = p->Nt;
+  for (uint32_t d = 1; d < p->D; ++d) V *= p->Nx;
+  if (V * p->D >= (1ull << 31)) return fail(DSA_E_RANGE, "V*D must be below 2^31 links");
+  if (p->rng != DSA_RNG_PHILOX && p->rng != DSA_RNG_MINSTD) return fail(DSA_E_ARG, "unknown rng");
+  if (p->order != DSA_ORDER_COLOUR && p->order != DSA_ORDER_REFERENCE) return fail(DSA_E_ARG, "unknown order");
+  if (p->start != DSA_START_HOT && p->start != DSA_START_COLD) return fail(DSA_E_ARG, "unknown start");
+  return 0;
+}
+
+void dsa::minstd_start(const Group &g, const dsa_chain_params &p, uint32_t V, std::vector<uint32_t> &ms,
+                       std::vector<int32_t> &a) {
+  ms.resize(V);
+  for (uint32_t i = 0; i < V; ++i) {
+    const uint64_t s = (uint64_t)(int64_t)(int32_t)((uint32_t)p.seed + i);
+    uint64_t x = s % 2147483647ull;
+    ms[i] = (uint32_t)(x == 0 ? 1 : x);
+  }
+  const size_t nl = (size_t)V * p.D;
+  a.assign(nl, g.id);
+  if (p.start == DSA_START_COLD) return;
+  // uniform_int_distribution(0, P-1) on minstd_rand0: two-division downscaling
+  const uint64_t urngrange = 2147483645ull, uerange = g.P;
+  const uint64_t scaling = urngrange / uerange, past = uerange * scaling;
+  for (size_t l = 0; l < nl; ++l) {
+    uint64_t r;
+    do r = (uint64_t)dsa::minstd_next(ms[0]) - 1;
+    while (r >= past);
+    a[l] = (int32_t)(r / scaling);
+  }
+}
+
+int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
   std::lock_guard<std::mutex> lk(g->m);
   auto it = g->dev.find(device);
   if (it != g->dev.end()) {
@@ -97,6 +134,8 @@ int upload_tables(dsa_group *g, int device, DeviceTables &out) {
   out = t;
   return 0;
 }
+
+namespace {
 
 void free_tables(DeviceTables &t) {
   if (t.mult) (void)hipFree(t.mult);
@@ -207,14 +246,7 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
   if (!gc || !p || !out) return fail(DSA_E_ARG, "dsa_chain_create: NULL argument");
   *out = nullptr;
   dsa_group *g = const_cast<dsa_group *>(gc);
-  if (p->D < 2 || p->D > (uint32_t)dsa::kDMax)
-    return fail(DSA_E_RANGE, "D = " + std::to_string(p->D) + " outside the supported 2..5");
-  if (p->Nt < 1 || p->Nx < 1) return fail(DSA_E_ARG, "Nt and Nx must be positive");
-  uint64_t V = p->Nt;
-  for (uint32_t d = 1; d < p->D; ++d) V *= p->Nx;
-  if (V * p->D >= (1ull << 31)) return fail(DSA_E_RANGE, "V*D must be below 2^31 links");
-  if (p->rng != DSA_RNG_PHILOX && p->rng != DSA_RNG_MINSTD) return fail(DSA_E_ARG, "unknown rng");
-  if (p->order != DSA_ORDER_COLOUR && p->order != DSA_ORDER_REFERENCE) return fail(DSA_E_ARG, "unknown order");
+  if (int rc0 = dsa::check_params(p)) return rc0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DSA_E_DEVICE, "no HIP device visible");
 
@@ -227,16 +259,9 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
     return code;
   };
   if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(DSA_E_DEVICE, "hipGetDevice failed"));
-  int rc = upload_tables(g, c->device, c->dt);
+  int rc = dsa::upload_tables(g, c->device, c->dt);
   if (rc) return bail(rc);
-  c->T.mult = c->dt.mult;
-  c->T.inv = c->dt.inv;
-  c->T.retr = c->dt.retr;
-  c->T.imtr = c->dt.imtr;
-  c->T.pool = c->dt.pool;
-  c->T.P = g->g.P;
-  c->T.npool = (uint32_t)g->g.pool.size();
-  c->T.id = (uint32_t)g->g.id;
+  c->T = dsa::tables_view(g, c->dt);
   if (dsa::sweep_lds_bytes(c->T) > 65536)
     return bail(fail(DSA_E_RANGE, "group order " + std::to_string(c->T.P) + " needs more than 64 KB of LDS tables"));
 
@@ -262,23 +287,9 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
   if (p->rng == DSA_RNG_MINSTD) {
     // dym-mod-metro.cpp:227-228, 253-255: per-site engines seeded iseed+i; hot
     // links drawn in link order from site 0's engine (host: it is sequential).
-    std::vector<uint32_t> ms(c->G.V);
-    for (uint32_t i = 0; i < c->G.V; ++i) {
-      const uint64_t s = (uint64_t)(int64_t)(int32_t)((uint32_t)p->seed + i);
-      uint64_t x = s % 2147483647ull;
-      ms[i] = (uint32_t)(x == 0 ? 1 : x);
-    }
-    std::vector<int32_t> a(nl, g->g.id);
-    if (!cold) {
-      const uint64_t urngrange = 2147483645ull, uerange = g->g.P;
-      const uint64_t scaling = urngrange / uerange, past = uerange * scaling;
-      for (size_t l = 0; l < nl; ++l) {
-        uint64_t r;
-        do r = (uint64_t)dsa::minstd_next(ms[0]) - 1;
-        while (r >= past);
-        a[l] = (int32_t)(r / scaling);
-      }
-    }
+    std::vector<uint32_t> ms;
+    std::vector<int32_t> a;
+    dsa::minstd_start(g->g, *p, c->G.V, ms, a);
     if (hipMalloc(&c->d_mstate, c->G.V * sizeof(uint32_t)) != hipSuccess ||
         hipMemcpy(c->d_mstate, ms.data(), c->G.V * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(DSA_E_DEVICE, "minstd state upload failed"));
@@ -303,6 +314,7 @@ void dsa_chain_destroy(dsa_chain *c) {
     if (ptr) (void)hipFree(ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->pev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -321,16 +333,36 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.beta1 = c->p.beta1;
   A.beta2 = c->p.beta2;
   A.seed = (uint32_t)c->p.seed;
+  const size_t np = c->S.passes.size();
+  if (c->profiling && c->pev.size() < 2 * np) {
+    for (size_t q = c->pev.size(); q < 2 * np; ++q) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      c->pev.push_back(e);
+    }
+  }
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   for (uint32_t it = 0; it < n; ++it) {
     A.sweep_lo = (uint32_t)c->sweep;
     A.sweep_hi = (uint32_t)(c->sweep >> 32);
-    for (const dsa::Pass &ps : c->S.passes) {
+    for (size_t q = 0; q < np; ++q) {
+      const dsa::Pass &ps = c->S.passes[q];
       A.mu = ps.mu;
       A.sites = c->d_sites + ps.offset;
       A.count = ps.count;
+      if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q], c->stream));
       HIP_TRY(dsa::launch_sweep(A, c->max_blocks, c->stream));
+      if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q + 1], c->stream));
       c->hit += (uint64_t)ps.count * dsa::kNHit;
+    }
+    if (c->profiling) {
+      HIP_TRY(hipEventSynchronize(c->pev[2 * np - 1]));
+      for (size_t q = 0; q < np; ++q) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, c->pev[2 * q], c->pev[2 * q + 1]));
+        c->kern_ms += t;
+      }
+      c->kern_launches += np;
     }
     c->sweep++;
   }
@@ -518,6 +550,81 @@ int dsa_schedule_query(uint32_t D, uint32_t Nt, uint32_t Nx, int32_t order, uint
 int dsa_chain_last_sweep_ms(dsa_chain *c, double *ms) {
   if (!c || !ms) return fail(DSA_E_ARG, "dsa_chain_last_sweep_ms: NULL argument");
   *ms = c->last_ms;
+  return 0;
+}
+
+int dsa_chain_set_profiling(dsa_chain *c, int32_t on) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_set_profiling: NULL chain");
+  c->profiling = on != 0;
+  c->kern_ms = 0;
+  c->kern_launches = 0;
+  return 0;
+}
+
+int dsa_chain_kernel_time(dsa_chain *c, double *ms, uint64_t *launches) {
+  if (!c || !ms || !launches) return fail(DSA_E_ARG, "dsa_chain_kernel_time: NULL argument");
+  *ms = c->kern_ms;
+  *launches = c->kern_launches;
+  return 0;
+}
+
+int dsa_probe_gather(const dsa_group *gc, uint32_t blocks, uint32_t rounds, double *gathers_per_s) {
+  if (!gc || !gathers_per_s || !blocks || !rounds) return fail(DSA_E_ARG, "dsa_probe_gather: bad argument");
+  dsa_group *g = const_cast<dsa_group *>(gc);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  DeviceTables t;
+  int rc = dsa::upload_tables(g, dev, t);
+  if (rc) return rc;
+  uint32_t *sink = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipMalloc(&sink, (size_t)blocks * 256 * sizeof(uint32_t)));
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipError_t e = dsa::launch_gather_probe(t.mult, g->g.P, blocks, 2, sink, s);  // warm the table into L2
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  if (e == hipSuccess) e = dsa::launch_gather_probe(t.mult, g->g.P, blocks, rounds, sink, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  (void)hipFree(sink);
+  if (e != hipSuccess) return fail(DSA_E_DEVICE, std::string("gather probe: ") + hipGetErrorString(e));
+  *gathers_per_s = (double)blocks * 256.0 * rounds * 24.0 / (ms * 1e-3);
+  return 0;
+}
+
+int dsa_probe_copy(uint64_t bytes, uint32_t reps, double *ms_per_copy) {
+  if (!ms_per_copy || bytes < 16 || !reps) return fail(DSA_E_ARG, "dsa_probe_copy: bad argument");
+  bytes &= ~(uint64_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipMalloc(&a, bytes));
+  HIP_TRY(hipMalloc(&b, bytes));
+  HIP_TRY(hipMemset(a, 1, bytes));
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipError_t e = dsa::launch_copy_probe(a, b, bytes, s);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) e = dsa::launch_copy_probe(a, b, bytes, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(DSA_E_DEVICE, std::string("copy probe: ") + hipGetErrorString(e));
+  *ms_per_copy = ms / reps;
   return 0;
 }
 

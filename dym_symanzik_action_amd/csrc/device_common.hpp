@@ -1,0 +1,354 @@
+// device_common.hpp -- device helpers shared by the gfx950 kernel files:
+// lattice geometry, link addressing, staple products, closed loops and the
+// fixed-shape reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dsa_math.hpp"
+#include "kernels.hpp"
+
+namespace dsa {
+namespace {
+
+constexpr int kBlock = 256;
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD one
+// contiguous range of the pass so neighbouring links share an L2.
+__device__ inline uint32_t xcd_block() {
+  const uint32_t b = blockIdx.x, n = gridDim.x;
+  if ((n & 7u) == 0) return (b & 7u) * (n >> 3) + (b >> 3);
+  return b;
+}
+
+__device__ inline int wrapc(int v, int n) {
+  if (v < 0) v += n;
+  if (v < 0) v += n;
+  if (v >= n) v -= n;
+  if (v >= n) v -= n;
+  return v;
+}
+
+template <int D>
+struct Site {
+  int x[D];
+  uint32_t i;
+  uint32_t par;
+};
+
+template <int D>
+__device__ inline Site<D> decode(const Geom &G, uint32_t i) {
+  Site<D> s;
+  s.i = i;
+  uint32_t par = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const uint32_t q = i / G.ext[d];
+    s.x[d] = (int)(i - q * G.ext[d]);
+    par += (uint32_t)s.x[d];
+    i = q;
+  }
+  s.par = par & 1u;
+  return s;
+}
+
+// s.x[d] for a run-time d without indexing a register array (no scratch).
+template <int D>
+__device__ inline int coord(const Site<D> &s, int d) {
+  int r = s.x[0];
+#pragma unroll
+  for (int q = 1; q < D; ++q) r = (q == d) ? s.x[q] : r;
+  return r;
+}
+
+// Site at x + a e_mu + b e_nu (periodic); its parity in *par.
+template <int D>
+__device__ inline uint32_t shifted(const Geom &G, const Site<D> &s, int mu, int a, int nu, int b, uint32_t *par) {
+  uint32_t i = s.i;
+  uint32_t p = s.par;
+  if (a) {
+    const int x = coord<D>(s, mu), n = (int)G.ext[mu];
+    const int y = wrapc(x + a, n);
+    i += (uint32_t)(y - x) * G.stride[mu];
+    p ^= (uint32_t)(y - x) & 1u;
+  }
+  if (b) {
+    const int x = coord<D>(s, nu), n = (int)G.ext[nu];
+    const int y = wrapc(x + b, n);
+    i += (uint32_t)(y - x) * G.stride[nu];
+    p ^= (uint32_t)(y - x) & 1u;
+  }
+  *par = p;
+  return i;
+}
+
+__device__ inline uint32_t lpos(const Geom &G, uint32_t site, uint32_t par) {
+  return G.eo ? (par ? G.halfV : 0u) + (site >> 1) : site;
+}
+
+// Link U_dir(x + a e_mu + b e_nu).
+template <int D>
+__device__ inline uint32_t U(const uint16_t *links, const Geom &G, const Site<D> &s, int dir, int mu,
+                             int a, int nu, int b) {
+  uint32_t par;
+  const uint32_t site = shifted<D>(G, s, mu, a, nu, b, &par);
+  return links[(size_t)dir * G.V + lpos(G, site, par)];
+}
+
+struct TabView {
+  const uint16_t *__restrict__ mult;
+  const uint16_t *inv;  // LDS or global
+  uint32_t P;
+  __device__ inline uint32_t M(uint32_t a, uint32_t b) const { return mult[a * P + b]; }
+  __device__ inline uint32_t I(uint32_t a) const { return inv[a]; }
+};
+
+// The 2(D-1) plaquette and 6(D-1) rectangle staples of link (x, mu), in the
+// reference's order and with its products: dym-mod-metro.cpp:63-78 and
+// 83-142.  For each transverse nu (ascending): up, down plaquette staples;
+// rectangles k0 (1x2 up), k1 (2x1 forward-up), k2 (2x1 backward-up), k3 (1x2
+// down), k4 (2x1 forward-down), k5 (2x1 backward-down).  The reference's
+// leading mult[id][.] is the identity map and is skipped.
+template <int D>
+__device__ inline void build_staples(const uint16_t *links, const Geom &G, const TabView &T,
+                                     const Site<D> &s, int mu, uint32_t *st, uint32_t *re) {
+  const uint32_t umu_p = U<D>(links, G, s, mu, mu, 1, mu, 0);   // U_mu(x+mu)
+  const uint32_t umu_m = U<D>(links, G, s, mu, mu, -1, mu, 0);  // U_mu(x-mu)
+#pragma unroll
+  for (int j = 0; j < D - 1; ++j) {
+    const int nu = j + (j >= mu ? 1 : 0);
+    // U_nu at x + a mu + b nu
+    const uint32_t n_10 = U<D>(links, G, s, nu, mu, 1, nu, 0);
+    const uint32_t n_00 = U<D>(links, G, s, nu, mu, 0, nu, 0);
+    const uint32_t n_1m = U<D>(links, G, s, nu, mu, 1, nu, -1);
+    const uint32_t n_0m = U<D>(links, G, s, nu, mu, 0, nu, -1);
+    const uint32_t n_11 = U<D>(links, G, s, nu, mu, 1, nu, 1);
+    const uint32_t n_01 = U<D>(links, G, s, nu, mu, 0, nu, 1);
+    const uint32_t n_20 = U<D>(links, G, s, nu, mu, 2, nu, 0);
+    const uint32_t n_m0 = U<D>(links, G, s, nu, mu, -1, nu, 0);
+    const uint32_t n_1M = U<D>(links, G, s, nu, mu, 1, nu, -2);
+    const uint32_t n_0M = U<D>(links, G, s, nu, mu, 0, nu, -2);
+    const uint32_t n_2m = U<D>(links, G, s, nu, mu, 2, nu, -1);
+    const uint32_t n_mm = U<D>(links, G, s, nu, mu, -1, nu, -1);
+    // U_mu at x + a mu + b nu
+    const uint32_t m_01 = U<D>(links, G, s, mu, mu, 0, nu, 1);
+    const uint32_t m_0m = U<D>(links, G, s, mu, mu, 0, nu, -1);
+    const uint32_t m_02 = U<D>(links, G, s, mu, mu, 0, nu, 2);
+    const uint32_t m_11 = U<D>(links, G, s, mu, mu, 1, nu, 1);
+    const uint32_t m_m1 = U<D>(links, G, s, mu, mu, -1, nu, 1);
+    const uint32_t m_0M = U<D>(links, G, s, mu, mu, 0, nu, -2);
+    const uint32_t m_1m = U<D>(links, G, s, mu, mu, 1, nu, -1);
+    const uint32_t m_mm = U<D>(links, G, s, mu, mu, -1, nu, -1);
+
+    // plaquette up: U_nu(x+mu) U_mu(x+nu)^-1 U_nu(x)^-1
+    st[2 * j] = T.M(T.M(n_10, T.I(m_01)), T.I(n_00));
+    // plaquette down: U_nu(x+mu-nu)^-1 U_mu(x-nu)^-1 U_nu(x-nu)
+    st[2 * j + 1] = T.M(T.M(T.I(n_1m), T.I(m_0m)), n_0m);
+    // k0
+    re[6 * j + 0] = T.M(T.M(T.M(T.M(n_10, n_11), T.I(m_02)), T.I(n_01)), T.I(n_00));
+    // k1
+    re[6 * j + 1] = T.M(T.M(T.M(T.M(umu_p, n_20), T.I(m_11)), T.I(m_01)), T.I(n_00));
+    // k2
+    re[6 * j + 2] = T.M(T.M(T.M(T.M(n_10, T.I(m_01)), T.I(m_m1)), T.I(n_m0)), umu_m);
+    // k3
+    re[6 * j + 3] = T.M(T.M(T.M(T.M(T.I(n_1m), T.I(n_1M)), T.I(m_0M)), n_0M), n_0m);
+    // k4
+    re[6 * j + 4] = T.M(T.M(T.M(T.M(umu_p, T.I(n_2m)), T.I(m_1m)), T.I(m_0m)), n_0m);
+    // k5
+    re[6 * j + 5] = T.M(T.M(T.M(T.M(T.I(n_1m), T.I(m_0m)), T.I(m_mm)), n_mm), umu_m);
+  }
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Per-chain constants of one link update.
+struct LinkCtx {
+  uint16_t *links;
+  uint32_t *mstate;
+  const double *s_retr;   // LDS
+  const uint16_t *s_pool; // LDS
+  double npool_d;
+  double beta0, beta1, beta2;
+  uint32_t seed, sweep_lo, sweep_hi;
+  int rng;
+};
+
+// NHIT Metropolis hits on link (i, mu) (dym-mod-metro.cpp:158-192): staples
+// built once from the current field, action contributions r1 (plaquettes) and
+// m1 (rectangles) summed in the reference's order, proposals b -> b*g with g
+// from the pool, acceptance min(1, exp(-dS)) against a fresh uniform.  Returns
+// the number of accepted hits.
+template <int D>
+__device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const TabView &T, int mu, uint32_t i) {
+  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1);
+  const Site<D> s = decode<D>(G, i);
+  uint32_t st[NS], re[NR];
+  build_staples<D>(C.links, G, T, s, mu, st, re);
+  const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
+  uint32_t b = C.links[lp];
+  const uint32_t b0 = b;
+  double r1 = 0, m1 = 0;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) r1 += C.s_retr[T.M(b, st[j])];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) m1 += C.s_retr[T.M(b, re[q])];
+  uint32_t ms = 0;
+  if (C.rng == kRngMinstd) ms = C.mstate[i];
+  const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
+  uint32_t acc = 0;
+  for (int h = 0; h < kNHit; ++h) {
+    double u1, u2;
+    if (C.rng == kRngPhilox) {
+      const U4 w = philox4x32_10(U4{link, (uint32_t)h, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
+      u1 = u53(w.x, w.y);
+      u2 = u53(w.z, w.w);
+    } else {
+      u1 = minstd_canonical(ms);
+      u2 = minstd_canonical(ms);  // same draw order as the reference: proposal, then acceptance
+    }
+    const uint32_t bnew = T.M(b, C.s_pool[(uint32_t)(C.npool_d * u1)]);
+    double r1n = 0, m1n = 0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) r1n += C.s_retr[T.M(bnew, st[j])];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) m1n += C.s_retr[T.M(bnew, re[q])];
+    const double oldact = C.beta0 + C.beta1 * r1 + C.beta2 * m1;
+    const double newact = C.beta0 + C.beta1 * r1n + C.beta2 * m1n;
+    const double probrat = (newact <= oldact) ? 1.0 : dsa_exp(-(newact - oldact));
+    if (probrat > u2) {
+      b = bnew;
+      r1 = r1n;
+      m1 = m1n;
+      ++acc;
+    }
+  }
+  if (b != b0) C.links[lp] = (uint16_t)b;
+  if (C.rng == kRngMinstd) C.mstate[i] = ms;
+  return acc;
+}
+
+// Loads ReTr, the inverse table and the proposal pool of one group into LDS.
+__device__ inline void load_tables_lds(const Tables &Tb, double *s_retr, uint16_t *s_inv, uint16_t *s_pool) {
+  for (uint32_t t = threadIdx.x; t < Tb.P; t += blockDim.x) {
+    s_retr[t] = Tb.retr[t];
+    s_inv[t] = Tb.inv[t];
+  }
+  for (uint32_t t = threadIdx.x; t < Tb.npool; t += blockDim.x) s_pool[t] = Tb.pool[t];
+}
+
+
+__device__ inline uint32_t site_parity(const Geom &G, uint32_t i) {
+  uint32_t par = 0;
+  for (int d = 0; d < G.D; ++d) {
+    const uint32_t q = i / G.ext[d];
+    par += i - q * G.ext[d];
+    i = q;
+  }
+  return par & 1u;
+}
+
+// ---------------------------------------------------------------- loops
+
+__device__ inline uint32_t link_at(const uint16_t *links, const Geom &G, uint32_t site, int dir) {
+  return links[(size_t)dir * G.V + lpos(G, site, site_parity(G, site))];
+}
+
+__device__ inline uint32_t step_site(const Geom &G, uint32_t i, int d, int sgn) {
+  const uint32_t n = G.ext[d], st = G.stride[d];
+  const uint32_t x = (i / st) % n;
+  const uint32_t y = sgn > 0 ? (x + 1 == n ? 0 : x + 1) : (x == 0 ? n - 1 : x - 1);
+  return i + (y - x) * st;
+}
+
+// lattice.c:76-83
+__device__ inline uint32_t plaq_elem(const uint16_t *links, const Geom &G, const TabView &T, uint32_t i, int d1,
+                                     int d2) {
+  uint32_t g = link_at(links, G, i, d1);
+  g = T.M(g, link_at(links, G, step_site(G, i, d1, 1), d2));
+  g = T.M(g, T.I(link_at(links, G, step_site(G, i, d2, 1), d1)));
+  g = T.M(g, T.I(link_at(links, G, i, d2)));
+  return g;
+}
+
+// wilson(i, d1, d2, 2, 1), lattice.c:116-124
+__device__ inline uint32_t rect_elem(const uint16_t *links, const Geom &G, const TabView &T, uint32_t i, int d1,
+                                     int d2) {
+  const uint32_t x1 = step_site(G, i, d1, 1), x2 = step_site(G, x1, d1, 1);
+  const uint32_t x2u = step_site(G, x2, d2, 1);
+  const uint32_t x1u = step_site(G, x2u, d1, -1), x0u = step_site(G, x1u, d1, -1);
+  uint32_t g = link_at(links, G, i, d1);
+  g = T.M(g, link_at(links, G, x1, d1));
+  g = T.M(g, link_at(links, G, x2, d2));
+  g = T.M(g, T.I(link_at(links, G, x1u, d1)));
+  g = T.M(g, T.I(link_at(links, G, x0u, d1)));
+  g = T.M(g, T.I(link_at(links, G, i, d2)));
+  return g;
+}
+
+// lattice.c:86-94 (site i has x0 = 0)
+__device__ inline uint32_t poly_elem(const uint16_t *links, const Geom &G, const TabView &T, uint32_t i) {
+  uint32_t g = link_at(links, G, i, 0);
+  uint32_t j = i;
+  for (uint32_t t = 1; t < G.ext[0]; ++t) {
+    j = step_site(G, j, 0, 1);
+    g = T.M(g, link_at(links, G, j, 0));
+  }
+  return g;
+}
+
+__device__ inline size_t line_row(int d, uint32_t n, uint32_t nmax_t, uint32_t nmax_x) {
+  return d == 0 ? (size_t)(n - 1) : (size_t)nmax_t + (size_t)(d - 1) * nmax_x + (n - 1);
+}
+
+// wilson(i, d1, 0, K, T) = L_d1(x,K) L_0(x+K d1,T) L_d1(x+T 0,K)^-1 L_0(x,T)^-1
+// (the same element as the reference's link-by-link product: the table is a group).
+__device__ inline uint32_t wilson_elem(const uint16_t *lines, const Geom &G, const TabView &T, uint32_t i, int d1,
+                                       uint32_t K, uint32_t Tt, uint32_t nmax_t, uint32_t nmax_x) {
+  const size_t V = G.V;
+  const uint32_t xd1 = (i / G.stride[d1]) % G.ext[d1];
+  const uint32_t x0 = i % G.ext[0];
+  const uint32_t yd1 = (xd1 + K) % G.ext[d1];
+  const uint32_t y0 = (x0 + Tt) % G.ext[0];
+  const uint32_t iK = i + (yd1 - xd1) * G.stride[d1];
+  const uint32_t iT = i + (y0 - x0);
+  const uint32_t a = lines[line_row(d1, K, nmax_t, nmax_x) * V + i];
+  const uint32_t b = lines[line_row(0, Tt, nmax_t, nmax_x) * V + iK];
+  const uint32_t c = lines[line_row(d1, K, nmax_t, nmax_x) * V + iT];
+  const uint32_t e = lines[line_row(0, Tt, nmax_t, nmax_x) * V + i];
+  return T.M(T.M(T.M(a, b), T.I(c)), T.I(e));
+}
+
+
+// ------------------------------------------------------------ reductions
+
+// Fixed-shape block sum: wave64 shuffle tree, then the waves in order.
+__device__ inline double block_sum(double v, double *s_part) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) s_part[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r += s_part[q];
+  __syncthreads();
+  return r;
+}
+
+
+// Normalisations of dym-mod-metro.cpp:291,303,317 and lattice.c:110-111, in
+// the reference's unsigned integer arithmetic.
+__device__ inline double obs_norm(const Geom &G, uint32_t q) {
+  const uint32_t V = G.V, D = (uint32_t)G.D;
+  if (q < 2) return (double)(V / G.ext[0]);
+  if (q == 2) return (double)(V * D * (D - 1) / 2);
+  if (q == 3) return (double)(V * D * (D - 1));
+  return (double)(V * (D - 1));
+}
+
+
+}  // namespace
+}  // namespace dsa

@@ -111,4 +111,10 @@ DSA_HD inline double dsa_exp(double x) {
   return er * bits_to_double((uint64_t)(k + 1023) << 52);
 }
 
+// Energies of the LLR window are kept as integers in units of 2^-32, so the
+// sum of a pass's changes does not depend on the order of the additions.
+constexpr double kFixScale = 4294967296.0;
+DSA_HD inline long long to_fix(double x) { return (long long)(x * kFixScale + (x >= 0.0 ? 0.5 : -0.5)); }
+DSA_HD inline double from_fix(long long v) { return (double)v / kFixScale; }
+
 }  // namespace dsa

@@ -48,6 +48,94 @@ struct SweepArgs {
 size_t sweep_lds_bytes(const Tables &T);
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s);
 
+// ---- many independent chains in one launch per pass ----
+struct BatchChain {
+  uint16_t *links;
+  const uint32_t *sites;  // the chain's schedule (all passes concatenated)
+  uint32_t *mstate;       // minstd state or null
+  Tables T;
+  Geom G;
+  double beta0, beta1, beta2;
+  uint32_t seed;
+  int rng;
+  uint64_t sweep_base;    // sweeps done before joining the batch counter
+};
+
+// A block's work: `count` (<= 256) consecutive sites of one chain's pass.
+struct BatchTask {
+  uint32_t chain;
+  uint32_t offset;  // into the chain's sites
+  uint32_t count;
+  int32_t mu;
+};
+
+// A block's share of one chain's measurement: sites [site0, site0 + nsites).
+struct ObsTask {
+  uint32_t chain;
+  uint32_t site0;
+  uint32_t nsites;
+};
+
+// ---- LLR: energy-window constrained chains (llr.cpp, llr2d.cpp) ----
+enum LlrPhase : uint32_t { kLlrSeed = 0, kLlrRM = 1, kLlrDone = 2, kLlrIdle = 3 };
+
+// One pass of a chain's schedule: sites [offset, offset + count), direction mu.
+struct PassRange {
+  uint32_t offset, count;
+  int32_t mu;
+  uint32_t pad;
+};
+
+struct LlrChain {
+  BatchChain base;   // links, sites, tables, geometry, Philox seed
+  uint16_t *old;     // previous link values of the current pass, indexed like sites
+  long long *dE1;    // per link: proposed change of S1 (fixed point), indexed like sites
+  long long *dE2;    // per link: proposed change of S2
+  uint8_t *nacc;     // per link: accepted hits of its proposal
+  const PassRange *passes;
+  int axis;          // 1: window S1 (plaquettes), 2: window S2 (rectangles)
+  double bg;         // coupling of the other action
+  long long lo, hi;  // window [Ewin - hw, Ewin + hw] in fixed point
+  double Ewin, W;    // centre and width 2*hw (Robbins-Monro)
+  uint32_t K, NRM;   // constrained sweeps per RM iteration; iteration cap
+  double tol;        // stop once |delta a| < tol (0: run NRM iterations)
+  uint32_t cap;      // drive-in sweep cap
+  uint32_t npass;    // passes per sweep of this chain
+};
+
+struct LlrState {
+  long long S1, S2;  // running plaquette / rectangle actions (fixed point)
+  long long r1, r2;  // resync accumulators
+  double a;          // tilt a_n
+  double sumE;       // sum of the windowed energy over the iteration's sweeps
+  double last_da;    // |delta a| of the last RM step
+  double maxdrift;   // max |incremental - recomputed| energy
+  uint32_t phase, k, m, seed_sweeps, drive, resync, reached, pad;
+  uint64_t sweep;    // Philox sweep counter
+  unsigned long long acc, hit;
+};
+
+// Proposals: every link of pass `pass` of every active chain.
+hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const BatchTask *tasks, uint32_t ntasks,
+                           size_t lds_bytes, hipStream_t s);
+// Per chain (one wave each): accept the pass's link proposals in site order
+// while the running energy obeys the window (or, in drive-in, approaches it),
+// restoring the links whose proposal is refused.
+hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, uint32_t n, uint32_t pass, hipStream_t s);
+hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n, hipStream_t s);
+hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, uint32_t n, const ObsTask *tasks,
+                             uint32_t ntasks, hipStream_t s);
+
+hipError_t launch_sweep_batch(int D, const BatchChain *chains, const BatchTask *tasks, uint32_t ntasks,
+                              size_t lds_bytes, const unsigned long long *sweep_ctr, unsigned long long *acc,
+                              hipStream_t s);
+hipError_t launch_count_sweep(unsigned long long *sweep_ctr, hipStream_t s);
+// out[c*4 + q]: mean Re P, Im P, plaquette, 2x1 rectangle of chain c; partials
+// holds 4*ntasks doubles; chain c owns tasks [first[c], first[c] + n[c]).
+hipError_t launch_measure_batch(double *out, double *partials, const BatchChain *chains, const ObsTask *tasks,
+                                uint32_t ntasks, const uint32_t *first, const uint32_t *n, uint32_t nchains,
+                                hipStream_t s);
+
 // Philox hot start: link l (reference order l = i*D+mu) <- floor(P*u).
 hipError_t launch_init_philox(uint16_t *links, const Geom &G, uint32_t P, uint32_t seed, hipStream_t s);
 hipError_t launch_fill(uint16_t *links, const Geom &G, uint16_t v, hipStream_t s);

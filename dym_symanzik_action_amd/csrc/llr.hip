@@ -1,0 +1,339 @@
+// llr.hip -- energy-window constrained Metropolis and the Robbins-Monro
+// update of the LLR method (llr.cpp:178-250, llr2d.cpp:178-259), on the
+// device.
+//
+// The reference visits links serially; every hit is tested against the
+// window on a running energy.  Here the links of one colour pass (which share
+// no plaquette and no rectangle) first make their NHIT tilted hits
+// (weight exp(-a dE_w - bg dE_bg)) in parallel, each producing a proposal and
+// its energy change; then, per chain, the proposals are accepted in ascending
+// site order as long as the running energy stays in [E_lo, E_hi] (drive-in:
+// as long as it moves closer to the window centre), and refused links are
+// restored.  Each link's step is a reversible kernel restricted to the window
+// by rejection, so the sweep keeps the constrained measure.  Energies are
+// integers in units of 2^-32 so that sums do not depend on their order.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "dsa_math.hpp"
+#include "kernels.hpp"
+
+namespace dsa {
+
+namespace {
+
+__device__ inline long long wave_sum_i64(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__device__ inline bool llr_active(uint32_t phase) { return phase == kLlrSeed || phase == kLlrRM; }
+
+// drive-in: is the windowed energy outside the window at the start of a pass?
+__device__ inline bool llr_drive(const LlrChain &LC, const LlrState &S) {
+  const long long Ew = LC.axis == 1 ? S.S1 : S.S2;
+  return S.phase == kLlrSeed && (Ew < LC.lo || Ew > LC.hi);
+}
+
+__device__ inline long long llabs_(long long v) { return v < 0 ? -v : v; }
+
+// Acceptance of a link's change of the windowed energy from Eb to Ea.
+__device__ inline bool llr_rule(bool drive, long long lo, long long hi, long long Eb, long long Ea) {
+  if (drive) {
+    const long long mid = lo + (hi - lo) / 2;
+    return llabs_(Ea - mid) < llabs_(Eb - mid);
+  }
+  return Ea >= lo && Ea <= hi;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, LlrState *st, const BatchTask *tasks) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BatchTask tk = tasks[blockIdx.x];
+  const LlrChain &LC = chains[tk.chain];
+  const LlrState &S = st[tk.chain];
+  if (!llr_active(S.phase)) return;  // uniform over the block
+  const BatchChain &BC = LC.base;
+  const uint32_t P = BC.T.P;
+  double *s_retr = reinterpret_cast<double *>(smem);
+  uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_retr + P);
+  uint16_t *s_pool = s_inv + P;
+  load_tables_lds(BC.T, s_retr, s_inv, s_pool);
+  __syncthreads();
+
+  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1);
+  const TabView T{BC.T.mult, s_inv, P};
+  const Geom &G = BC.G;
+  const int axis = LC.axis;
+  const bool drive = llr_drive(LC, S);
+  const double sign = from_fix(axis == 1 ? S.S1 : S.S2) < LC.Ewin ? 1.0 : -1.0;
+  const double a = S.a, bg = LC.bg;
+  const uint32_t sw_lo = (uint32_t)S.sweep, sw_hi = (uint32_t)(S.sweep >> 32);
+  const double npool_d = (double)BC.T.npool;
+  for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x) {
+    const uint32_t idx = tk.offset + t;
+    const uint32_t i = BC.sites[idx];
+    const int mu = tk.mu;
+    const Site<D> s = decode<D>(G, i);
+    uint32_t stp[NS], re[NR];
+    build_staples<D>(BC.links, G, T, s, mu, stp, re);
+    const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
+    uint32_t b = BC.links[lp];
+    const uint32_t b0 = b;
+    double r1 = 0, m1 = 0;
+    long long q1 = 0, q2 = 0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const double v = s_retr[T.M(b, stp[j])];
+      r1 += v;
+      q1 += to_fix(v);
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const double v = s_retr[T.M(b, re[q])];
+      m1 += v;
+      q2 += to_fix(v);
+    }
+    const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
+    uint32_t nacc = 0;
+    for (int h = 0; h < kNHit; ++h) {
+      const U4 w = philox4x32_10(U4{link, (uint32_t)h, sw_lo, sw_hi}, BC.seed, kKeyLLR);
+      const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
+      const uint32_t bnew = T.M(b, s_pool[(uint32_t)(npool_d * u1)]);
+      double r1n = 0, m1n = 0;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) r1n += s_retr[T.M(bnew, stp[j])];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) m1n += s_retr[T.M(bnew, re[q])];
+      const double dS1 = r1n - r1, dS2 = m1n - m1;
+      const double dEw = axis == 1 ? dS1 : dS2;
+      const double dEbg = axis == 1 ? dS2 : dS1;
+      bool accept;
+      if (drive) {
+        accept = sign * dEw > 0.0;  // greedy move toward the window (llr.cpp:208-209)
+      } else {
+        accept = dsa_exp(-a * dEw - bg * dEbg) > u2;  // llr.cpp:211-212
+      }
+      if (accept) {
+        b = bnew;
+        r1 = r1n;
+        m1 = m1n;
+        ++nacc;
+      }
+    }
+    long long d1 = 0, d2 = 0;
+    if (b != b0) {
+      BC.links[lp] = (uint16_t)b;
+      long long p1 = 0, p2 = 0;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) p1 += to_fix(s_retr[T.M(b, stp[j])]);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) p2 += to_fix(s_retr[T.M(b, re[q])]);
+      d1 = p1 - q1;
+      d2 = p2 - q2;
+    }
+    LC.old[idx] = (uint16_t)b0;
+    LC.dE1[idx] = d1;
+    LC.dE2[idx] = d2;
+    LC.nacc[idx] = (uint8_t)nacc;
+  }
+}
+
+// One wave per chain.  Chunks of 64 proposals are accepted at once when every
+// prefix of the chunk obeys the rule; otherwise lane 0 walks the chunk.
+__global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
+  const uint32_t c = blockIdx.x;
+  const int lane = (int)threadIdx.x;
+  const LlrChain &LC = chains[c];
+  LlrState &S = st[c];
+  if (!llr_active(S.phase) || pass >= LC.npass) return;
+  const PassRange pr = LC.passes[pass];
+  const bool drive = llr_drive(LC, S);
+  const bool ax1 = LC.axis == 1;
+  const long long lo = LC.lo, hi = LC.hi;
+  long long E1 = S.S1, E2 = S.S2;
+  unsigned long long acc = 0;
+  const Geom &G = LC.base.G;
+  for (uint32_t base = 0; base < pr.count; base += 64) {
+    const uint32_t j = base + (uint32_t)lane;
+    const bool valid = j < pr.count;
+    const uint32_t idx = pr.offset + j;
+    const long long d1 = valid ? LC.dE1[idx] : 0;
+    const long long d2 = valid ? LC.dE2[idx] : 0;
+    const long long dw = ax1 ? d1 : d2;
+    const unsigned long long na = valid ? LC.nacc[idx] : 0;
+    long long p = dw;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long t = __shfl_up(p, o, 64);
+      if (lane >= o) p += t;
+    }
+    const long long Ew = ax1 ? E1 : E2;
+    const bool ok = !valid || llr_rule(drive, lo, hi, Ew + p - dw, Ew + p);
+    if (__all(ok)) {
+      E1 += wave_sum_i64(d1);
+      E2 += wave_sum_i64(d2);
+      E1 = __shfl(E1, 0, 64);
+      E2 = __shfl(E2, 0, 64);
+      acc += __shfl(wave_sum_u64(na), 0, 64);
+    } else {
+      if (lane == 0) {
+        const uint32_t n = pr.count - base < 64 ? pr.count - base : 64;
+        for (uint32_t q = 0; q < n; ++q) {
+          const uint32_t k = pr.offset + base + q;
+          const long long e1 = LC.dE1[k], e2 = LC.dE2[k];
+          const long long ew = ax1 ? e1 : e2;
+          const long long Ecur = ax1 ? E1 : E2;
+          if (llr_rule(drive, lo, hi, Ecur, Ecur + ew)) {
+            E1 += e1;
+            E2 += e2;
+            acc += LC.nacc[k];
+          } else {
+            const uint32_t site = LC.base.sites[k];
+            LC.base.links[(size_t)pr.mu * G.V + lpos(G, site, site_parity(G, site))] = LC.old[k];
+          }
+        }
+      }
+      E1 = __shfl(E1, 0, 64);
+      E2 = __shfl(E2, 0, 64);
+      acc = __shfl(acc, 0, 64);
+    }
+  }
+  if (lane == 0) {
+    S.S1 = E1;
+    S.S2 = E2;
+    S.acc += acc;
+  }
+}
+
+// End of a sweep: drive-in bookkeeping (llr.cpp:223-231) and the
+// Robbins-Monro step a += 12/(W^2 (m+1)) (<E> - Ewin) (llr.cpp:237-249).
+__global__ void k_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const LlrChain &LC = chains[c];
+  LlrState &S = st[c];
+  if (!llr_active(S.phase)) return;
+  S.sweep += 1;
+  S.hit += (unsigned long long)LC.base.G.V * (unsigned long long)LC.base.G.D * kNHit;
+  const long long Ew = LC.axis == 1 ? S.S1 : S.S2;
+  if (S.phase == kLlrSeed) {
+    S.seed_sweeps += 1;
+    if (Ew >= LC.lo && Ew <= LC.hi) {
+      S.phase = kLlrRM;
+      S.reached = 1;
+      S.k = 0;
+      S.m = 0;
+      S.sumE = 0;
+      S.resync = 1;
+    } else if (S.seed_sweeps >= LC.cap) {
+      S.phase = kLlrDone;
+      S.reached = 0;
+    }
+    return;
+  }
+  S.sumE += from_fix(Ew);
+  S.k += 1;
+  if (S.k == LC.K) {
+    const double a_old = S.a;
+    const double g = S.sumE / LC.K - LC.Ewin;
+    S.a += (12.0 / (LC.W * LC.W * (S.m + 1))) * g;
+    S.m += 1;
+    S.k = 0;
+    S.sumE = 0;
+    S.last_da = fabs(S.a - a_old);
+    S.resync = 1;
+    if ((LC.tol > 0.0 && S.last_da < LC.tol) || S.m >= LC.NRM) S.phase = kLlrDone;
+  }
+}
+
+// Recompute S1 = sum of plaquette ReTr and S2 = sum of 2x1 rectangle ReTr
+// (llr.cpp:46-55,139-151, each rectangle once) for the chains that asked
+// (resync 1: measure the drift first; 2: initial, no drift).
+__global__ void k_llr_resync_zero(LlrState *st, uint32_t n) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n && st[c].resync) {
+    st[c].r1 = 0;
+    st[c].r2 = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_llr_resync_sum(const LlrChain *chains, LlrState *st,
+                                                           const ObsTask *tasks) {
+  const ObsTask tk = tasks[blockIdx.x];
+  LlrState &S = st[tk.chain];
+  if (!S.resync) return;
+  const BatchChain &BC = chains[tk.chain].base;
+  const Geom G = BC.G;
+  const TabView T{BC.T.mult, BC.T.inv, BC.T.P};
+  const int D = G.D;
+  long long p1 = 0, p2 = 0;
+  for (uint32_t t = threadIdx.x; t < tk.nsites; t += blockDim.x) {
+    const uint32_t i = tk.site0 + t;
+    for (int d1 = 0; d1 < D; ++d1)
+      for (int d2 = d1 + 1; d2 < D; ++d2) p1 += to_fix(BC.T.retr[plaq_elem(BC.links, G, T, i, d1, d2)]);
+    for (int d1 = 0; d1 < D; ++d1)
+      for (int d2 = 0; d2 < D; ++d2)
+        if (d1 != d2) p2 += to_fix(BC.T.retr[rect_elem(BC.links, G, T, i, d1, d2)]);
+  }
+  p1 = wave_sum_i64(p1);
+  p2 = wave_sum_i64(p2);
+  if ((threadIdx.x & 63u) == 0) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(&S.r1), (unsigned long long)p1);
+    atomicAdd(reinterpret_cast<unsigned long long *>(&S.r2), (unsigned long long)p2);
+  }
+}
+
+__global__ void k_llr_resync_apply(const LlrChain *chains, LlrState *st, uint32_t n) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || !st[c].resync) return;
+  LlrState &S = st[c];
+  if (S.resync == 1) {
+    const bool ax1 = chains[c].axis == 1;
+    const double drift = fabs(from_fix(ax1 ? S.S1 : S.S2) - from_fix(ax1 ? S.r1 : S.r2));
+    if (drift > S.maxdrift) S.maxdrift = drift;
+  }
+  S.S1 = S.r1;
+  S.S2 = S.r2;
+  S.resync = 0;
+}
+
+inline dim3 lanes(uint32_t n) { return dim3((n + 63) / 64); }
+
+}  // namespace
+
+hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const BatchTask *tasks, uint32_t ntasks,
+                           size_t lds_bytes, hipStream_t s) {
+  if (!ntasks) return hipSuccess;
+  const dim3 g(ntasks), b(kBlock);
+  switch (D) {
+    case 2: hipLaunchKernelGGL(k_llr_pass<2>, g, b, lds_bytes, s, chains, st, tasks); break;
+    case 3: hipLaunchKernelGGL(k_llr_pass<3>, g, b, lds_bytes, s, chains, st, tasks); break;
+    case 4: hipLaunchKernelGGL(k_llr_pass<4>, g, b, lds_bytes, s, chains, st, tasks); break;
+    case 5: hipLaunchKernelGGL(k_llr_pass<5>, g, b, lds_bytes, s, chains, st, tasks); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, uint32_t n, uint32_t pass, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_llr_decide, dim3(n), dim3(64), 0, s, chains, st, pass);
+  return hipGetLastError();
+}
+
+hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_llr_sweep_end, lanes(n), dim3(64), 0, s, chains, st, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, uint32_t n, const ObsTask *tasks,
+                             uint32_t ntasks, hipStream_t s) {
+  hipLaunchKernelGGL(k_llr_resync_zero, lanes(n), dim3(64), 0, s, st, n);
+  if (ntasks) hipLaunchKernelGGL(k_llr_resync_sum, dim3(ntasks), dim3(kBlock), 0, s, chains, st, tasks);
+  hipLaunchKernelGGL(k_llr_resync_apply, lanes(n), dim3(64), 0, s, chains, st, n);
+  return hipGetLastError();
+}
+
+}  // namespace dsa

@@ -1,0 +1,420 @@
+// llr_engine.cpp -- device-resident LLR chains (include/dsa.h, dsa_llr_*):
+// admission into slots, one launch per pass over every active chain, the
+// per-chain drive-in / Robbins-Monro state machine on the device, and a
+// continuous-batching serving loop for streams of window requests.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dsa_math.hpp"
+#include "internal.hpp"
+#include "schedule.hpp"
+
+using dsa::fail;
+
+namespace {
+
+constexpr uint32_t kTaskLinks = 256;
+constexpr uint32_t kObsSites = 1024;
+
+struct Slot {
+  bool used = false;
+  dsa_group *grp = nullptr;
+  dsa_llr_params p{};
+  dsa::Geom G{};
+  dsa::Schedule S;
+  dsa::DeviceTables dt;
+  uint16_t *links = nullptr, *old = nullptr;
+  uint32_t *sites = nullptr;
+  long long *dE1 = nullptr, *dE2 = nullptr;
+  uint8_t *nacc = nullptr;
+  dsa::PassRange *passes = nullptr;
+  uint64_t bytes = 0;
+};
+
+// links, previous values, site lists, two energy changes and a hit count per link
+uint64_t footprint(const dsa_llr_params &p) {
+  uint64_t V = p.Nt;
+  for (uint32_t d = 1; d < p.D; ++d) V *= p.Nx;
+  const uint64_t nl = V * p.D;
+  return nl * (2 + 2 + 4 + 8 + 8 + 1) + 64 * sizeof(dsa::PassRange) + sizeof(dsa::LlrChain) +
+         sizeof(dsa::LlrState);
+}
+
+}  // namespace
+
+struct dsa_llr {
+  int D = 0;
+  int device = 0;
+  uint32_t cap = 0;
+  uint64_t budget = 0, used_bytes = 0;
+  std::vector<Slot> slots;
+  hipStream_t stream = nullptr;
+  dsa::LlrChain *d_chains = nullptr;
+  dsa::LlrState *d_state = nullptr;
+  dsa::BatchTask *d_tasks = nullptr;
+  dsa::ObsTask *d_obs = nullptr;
+  size_t tasks_cap = 0, obs_cap = 0;
+  std::vector<uint32_t> pass_off, pass_n;
+  uint32_t nobs = 0;
+  size_t lds = 0;
+  bool dirty = true;
+  std::vector<dsa::LlrState> host_state;
+};
+
+namespace {
+
+int rebuild_tasks(dsa_llr *L) {
+  std::vector<dsa::BatchTask> tasks;
+  std::vector<dsa::ObsTask> obs;
+  size_t maxpass = 0;
+  for (const Slot &s : L->slots)
+    if (s.used) maxpass = std::max(maxpass, s.S.passes.size());
+  L->pass_off.clear();
+  L->pass_n.clear();
+  for (size_t q = 0; q < maxpass; ++q) {
+    L->pass_off.push_back((uint32_t)tasks.size());
+    for (uint32_t c = 0; c < L->slots.size(); ++c) {
+      const Slot &s = L->slots[c];
+      if (!s.used || q >= s.S.passes.size()) continue;
+      const dsa::Pass &ps = s.S.passes[q];
+      for (uint32_t o = 0; o < ps.count; o += kTaskLinks)
+        tasks.push_back(dsa::BatchTask{c, ps.offset + o, std::min(kTaskLinks, ps.count - o), ps.mu});
+    }
+    L->pass_n.push_back((uint32_t)tasks.size() - L->pass_off.back());
+  }
+  for (uint32_t c = 0; c < L->slots.size(); ++c) {
+    const Slot &s = L->slots[c];
+    if (!s.used) continue;
+    for (uint32_t s0 = 0; s0 < s.G.V; s0 += kObsSites)
+      obs.push_back(dsa::ObsTask{c, s0, std::min(kObsSites, s.G.V - s0)});
+  }
+  if (tasks.size() > L->tasks_cap) {
+    if (L->d_tasks) HIP_TRY(hipFree(L->d_tasks));
+    L->d_tasks = nullptr;
+    L->tasks_cap = tasks.size() * 2;
+    HIP_TRY(hipMalloc(&L->d_tasks, L->tasks_cap * sizeof(dsa::BatchTask)));
+  }
+  if (obs.size() > L->obs_cap) {
+    if (L->d_obs) HIP_TRY(hipFree(L->d_obs));
+    L->d_obs = nullptr;
+    L->obs_cap = obs.size() * 2;
+    HIP_TRY(hipMalloc(&L->d_obs, L->obs_cap * sizeof(dsa::ObsTask)));
+  }
+  if (!tasks.empty())
+    HIP_TRY(hipMemcpyAsync(L->d_tasks, tasks.data(), tasks.size() * sizeof(dsa::BatchTask), hipMemcpyHostToDevice,
+                           L->stream));
+  if (!obs.empty())
+    HIP_TRY(hipMemcpyAsync(L->d_obs, obs.data(), obs.size() * sizeof(dsa::ObsTask), hipMemcpyHostToDevice,
+                           L->stream));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  L->nobs = (uint32_t)obs.size();
+  L->dirty = false;
+  return 0;
+}
+
+void release(Slot &s) {
+  for (void *p : {(void *)s.links, (void *)s.old, (void *)s.sites, (void *)s.dE1, (void *)s.dE2, (void *)s.nacc,
+                  (void *)s.passes})
+    if (p) (void)hipFree(p);
+  s = Slot();
+}
+
+int step(dsa_llr *L, uint32_t n) {
+  if (L->dirty)
+    if (int rc = rebuild_tasks(L)) return rc;
+  const uint32_t nc = (uint32_t)L->slots.size();
+  HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nc, L->d_obs, L->nobs, L->stream));
+  for (uint32_t it = 0; it < n; ++it) {
+    for (size_t q = 0; q < L->pass_off.size(); ++q) {
+      const dsa::BatchTask *tk = L->d_tasks + L->pass_off[q];
+      HIP_TRY(dsa::launch_llr_pass(L->D, L->d_chains, L->d_state, tk, L->pass_n[q], L->lds, L->stream));
+      HIP_TRY(dsa::launch_llr_decide(L->d_chains, L->d_state, nc, (uint32_t)q, L->stream));
+    }
+    HIP_TRY(dsa::launch_llr_sweep_end(L->d_chains, L->d_state, nc, L->stream));
+    HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nc, L->d_obs, L->nobs, L->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int pull_state(dsa_llr *L) {
+  L->host_state.resize(L->slots.size());
+  HIP_TRY(hipMemcpy(L->host_state.data(), L->d_state, L->slots.size() * sizeof(dsa::LlrState),
+                    hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsa_llr_footprint(const dsa_llr_params *p, uint64_t *bytes) {
+  if (!p || !bytes) return fail(DSA_E_ARG, "dsa_llr_footprint: NULL argument");
+  *bytes = footprint(*p);
+  return 0;
+}
+
+int dsa_llr_create(uint32_t D, uint32_t max_chains, uint64_t hbm_budget, dsa_llr **out) {
+  if (!out || !max_chains || D < 2 || D > (uint32_t)dsa::kDMax) return fail(DSA_E_ARG, "dsa_llr_create: bad argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DSA_E_DEVICE, "no HIP device visible");
+  auto *L = new dsa_llr();
+  L->D = (int)D;
+  L->cap = max_chains;
+  L->budget = hbm_budget;
+  L->slots.resize(max_chains);
+  auto bail = [&](int rc) {
+    dsa_llr_destroy(L);
+    return rc;
+  };
+  if (hipGetDevice(&L->device) != hipSuccess) return bail(fail(DSA_E_DEVICE, "hipGetDevice failed"));
+  if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&L->d_chains, max_chains * sizeof(dsa::LlrChain)) != hipSuccess ||
+      hipMalloc(&L->d_state, max_chains * sizeof(dsa::LlrState)) != hipSuccess)
+    return bail(fail(DSA_E_DEVICE, "allocation failed"));
+  std::vector<dsa::LlrState> idle(max_chains);
+  memset(idle.data(), 0, idle.size() * sizeof(dsa::LlrState));
+  for (auto &s : idle) s.phase = dsa::kLlrIdle;
+  std::vector<dsa::LlrChain> zero(max_chains);
+  memset(zero.data(), 0, zero.size() * sizeof(dsa::LlrChain));
+  if (hipMemcpy(L->d_state, idle.data(), idle.size() * sizeof(dsa::LlrState), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(L->d_chains, zero.data(), zero.size() * sizeof(dsa::LlrChain), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(DSA_E_DEVICE, "upload failed"));
+  *out = L;
+  return 0;
+}
+
+void dsa_llr_destroy(dsa_llr *L) {
+  if (!L) return;
+  (void)hipSetDevice(L->device);
+  if (L->stream) (void)hipStreamSynchronize(L->stream);
+  for (Slot &s : L->slots) release(s);
+  for (void *p : {(void *)L->d_chains, (void *)L->d_state, (void *)L->d_tasks, (void *)L->d_obs})
+    if (p) (void)hipFree(p);
+  if (L->stream) (void)hipStreamDestroy(L->stream);
+  delete L;
+}
+
+int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const int32_t *init_links,
+                uint32_t *slot_out) {
+  if (!L || !gc || !p) return fail(DSA_E_ARG, "dsa_llr_add: NULL argument");
+  if ((int)p->D != L->D) return fail(DSA_E_ARG, "chain dimension differs from the engine's");
+  dsa_chain_params cp{p->D, p->Nt, p->Nx, 0, 0, 0, p->seed, p->start, DSA_RNG_PHILOX, DSA_ORDER_COLOUR};
+  if (int rc = dsa::check_params(&cp)) return rc;
+  if (p->axis != 1 && p->axis != 2) return fail(DSA_E_ARG, "axis must be 1 or 2");
+  if (!(p->hw > 0) || !p->K || !p->NRM) return fail(DSA_E_ARG, "need hw > 0, K > 0, NRM > 0");
+  const uint64_t bytes = footprint(*p);
+  if (L->budget && L->used_bytes + bytes > L->budget) return fail(DSA_E_RANGE, "HBM budget exhausted");
+  uint32_t c = 0;
+  while (c < L->slots.size() && L->slots[c].used) ++c;
+  if (c == L->slots.size()) return fail(DSA_E_RANGE, "no free slot");
+  HIP_TRY(hipSetDevice(L->device));
+  dsa_group *g = const_cast<dsa_group *>(gc);
+  Slot s;
+  s.grp = g;
+  s.p = *p;
+  s.G = dsa::make_geom(p->D, p->Nt, p->Nx);
+  s.bytes = bytes;
+  if (int rc = dsa::upload_tables(g, L->device, s.dt)) return rc;
+  const dsa::Tables T = dsa::tables_view(g, s.dt);
+  if (dsa::sweep_lds_bytes(T) > 65536) return fail(DSA_E_RANGE, "group order too large for the LDS tables");
+  std::string err = dsa::build_colour_schedule(s.G, s.S);
+  if (!err.empty()) return fail(DSA_E_STATE, err);
+  const size_t nl = (size_t)s.G.V * s.G.D;
+  auto cleanup = [&](int rc) {
+    release(s);
+    return rc;
+  };
+  const size_t ns = s.S.sites.size();
+  std::vector<dsa::PassRange> pr;
+  for (const dsa::Pass &ps : s.S.passes) pr.push_back(dsa::PassRange{ps.offset, ps.count, ps.mu, 0});
+  if (hipMalloc(&s.links, nl * sizeof(uint16_t)) != hipSuccess || hipMalloc(&s.old, ns * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&s.sites, ns * sizeof(uint32_t)) != hipSuccess || hipMalloc(&s.dE1, ns * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&s.dE2, ns * sizeof(long long)) != hipSuccess || hipMalloc(&s.nacc, ns) != hipSuccess ||
+      hipMalloc(&s.passes, pr.size() * sizeof(dsa::PassRange)) != hipSuccess)
+    return cleanup(fail(DSA_E_DEVICE, "chain allocation failed"));
+  if (hipMemcpyAsync(s.sites, s.S.sites.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, L->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(s.passes, pr.data(), pr.size() * sizeof(dsa::PassRange), hipMemcpyHostToDevice, L->stream) !=
+          hipSuccess)
+    return cleanup(fail(DSA_E_DEVICE, "upload failed"));
+  hipError_t e;
+  int32_t *tmp = nullptr;
+  if (init_links) {
+    for (size_t l = 0; l < nl; ++l)
+      if (init_links[l] < 0 || (uint32_t)init_links[l] >= g->g.P) return cleanup(fail(DSA_E_ARG, "link out of range"));
+    e = hipMalloc(&tmp, nl * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp, init_links, nl * sizeof(int32_t), hipMemcpyHostToDevice, L->stream);
+    if (e == hipSuccess) e = dsa::launch_pack(s.links, tmp, s.G, L->stream);
+  } else if (p->start == DSA_START_COLD) {
+    e = dsa::launch_fill(s.links, s.G, (uint16_t)g->g.id, L->stream);
+  } else {
+    e = dsa::launch_init_philox(s.links, s.G, g->g.P, (uint32_t)p->seed, L->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  if (tmp) (void)hipFree(tmp);
+  if (e != hipSuccess) return cleanup(fail(DSA_E_DEVICE, std::string("chain start: ") + hipGetErrorString(e)));
+
+  dsa::LlrChain d{};
+  d.base.links = s.links;
+  d.base.sites = s.sites;
+  d.base.mstate = nullptr;
+  d.base.T = T;
+  d.base.G = s.G;
+  d.base.seed = (uint32_t)p->seed;
+  d.base.rng = dsa::kRngPhilox;
+  d.old = s.old;
+  d.dE1 = s.dE1;
+  d.dE2 = s.dE2;
+  d.nacc = s.nacc;
+  d.passes = s.passes;
+  d.axis = p->axis;
+  d.bg = p->bg;
+  d.lo = dsa::to_fix(p->Ewin - p->hw);
+  d.hi = dsa::to_fix(p->Ewin + p->hw);
+  d.Ewin = p->Ewin;
+  d.W = 2 * p->hw;
+  d.K = p->K;
+  d.NRM = p->NRM;
+  d.tol = p->tol;
+  d.cap = p->cap ? p->cap : 20000;
+  d.npass = (uint32_t)s.S.passes.size();
+  dsa::LlrState st{};
+  st.a = p->a0;
+  st.phase = dsa::kLlrSeed;
+  st.resync = 2;  // initial energies: computed, not compared
+  if (hipMemcpy(L->d_chains + c, &d, sizeof d, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(L->d_state + c, &st, sizeof st, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(DSA_E_DEVICE, "descriptor upload failed"));
+  s.used = true;
+  L->slots[c] = s;
+  L->used_bytes += bytes;
+  L->lds = std::max(L->lds, dsa::sweep_lds_bytes(T));
+  L->dirty = true;
+  if (slot_out) *slot_out = c;
+  return 0;
+}
+
+int dsa_llr_remove(dsa_llr *L, uint32_t slot) {
+  if (!L || slot >= L->slots.size() || !L->slots[slot].used) return fail(DSA_E_ARG, "dsa_llr_remove: bad slot");
+  HIP_TRY(hipSetDevice(L->device));
+  HIP_TRY(hipStreamSynchronize(L->stream));
+  dsa::LlrState idle{};
+  idle.phase = dsa::kLlrIdle;
+  HIP_TRY(hipMemcpy(L->d_state + slot, &idle, sizeof idle, hipMemcpyHostToDevice));
+  L->used_bytes -= L->slots[slot].bytes;
+  release(L->slots[slot]);
+  L->dirty = true;
+  return 0;
+}
+
+int dsa_llr_step(dsa_llr *L, uint32_t nsweeps) {
+  if (!L) return fail(DSA_E_ARG, "dsa_llr_step: NULL engine");
+  HIP_TRY(hipSetDevice(L->device));
+  return step(L, nsweeps);
+}
+
+int dsa_llr_status_get(dsa_llr *L, uint32_t slot, dsa_llr_status *out) {
+  if (!L || !out || slot >= L->slots.size() || !L->slots[slot].used)
+    return fail(DSA_E_ARG, "dsa_llr_status_get: bad slot");
+  HIP_TRY(hipSetDevice(L->device));
+  dsa::LlrState s;
+  HIP_TRY(hipMemcpy(&s, L->d_state + slot, sizeof s, hipMemcpyDeviceToHost));
+  out->a = s.a;
+  out->S1 = dsa::from_fix(s.S1);
+  out->S2 = dsa::from_fix(s.S2);
+  out->last_da = s.last_da;
+  out->maxdrift = s.maxdrift;
+  out->phase = s.phase;
+  out->iterations = s.m;
+  out->seed_sweeps = s.seed_sweeps;
+  out->reached = s.reached;
+  out->sweeps = s.sweep;
+  out->hit = s.hit;
+  out->acc = s.acc;
+  return 0;
+}
+
+int dsa_llr_get_links(dsa_llr *L, uint32_t slot, int32_t *a) {
+  if (!L || !a || slot >= L->slots.size() || !L->slots[slot].used)
+    return fail(DSA_E_ARG, "dsa_llr_get_links: bad slot");
+  HIP_TRY(hipSetDevice(L->device));
+  const Slot &s = L->slots[slot];
+  const size_t nl = (size_t)s.G.V * s.G.D;
+  int32_t *tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, nl * sizeof(int32_t)));
+  hipError_t e = dsa::launch_unpack(tmp, s.links, s.G, L->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(a, tmp, nl * sizeof(int32_t), hipMemcpyDeviceToHost, L->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(DSA_E_DEVICE, std::string("get links: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Serving: requests arrive at arrival_s[r] seconds after the call starts; each
+// is admitted into a free slot once it has arrived and its footprint fits the
+// budget, advanced `chunk` sweeps per round together with every other active
+// window, and retired as soon as its Robbins-Monro iteration converges (or its
+// drive-in fails).  done_s[r] is the retirement time, status[r] its final state.
+int dsa_llr_serve(dsa_llr *L, const dsa_group *const *groups, const dsa_llr_params *req, const double *arrival_s,
+                  uint32_t nreq, uint32_t chunk, double *admit_s, double *done_s, dsa_llr_status *status) {
+  if (!L || !groups || !req || !arrival_s || !done_s || !status || !nreq)
+    return fail(DSA_E_ARG, "dsa_llr_serve: NULL argument");
+  if (!chunk) chunk = 1;
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  auto now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+  std::vector<int> slot_of(nreq, -1), req_of(L->slots.size(), -1);
+  std::vector<uint32_t> order(nreq);
+  for (uint32_t r = 0; r < nreq; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return arrival_s[x] < arrival_s[y]; });
+  size_t next = 0;
+  uint32_t finished = 0, active = 0;
+  while (finished < nreq) {
+    // admit everything that has arrived and fits
+    while (next < nreq && arrival_s[order[next]] <= now()) {
+      const uint32_t r = order[next];
+      uint32_t s;
+      int rc = dsa_llr_add(L, groups[r], &req[r], nullptr, &s);
+      if (rc == DSA_E_RANGE && active) break;  // full: wait for a retirement
+      if (rc) return rc;                       // cannot fit even alone
+      slot_of[r] = (int)s;
+      req_of[s] = (int)r;
+      if (admit_s) admit_s[r] = now();
+      ++active;
+      ++next;
+    }
+    if (!active) {
+      if (next < nreq) {
+        const double wait = arrival_s[order[next]] - now();
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+      }
+      continue;
+    }
+    if (int rc = step(L, chunk)) return rc;
+    if (int rc = pull_state(L)) return rc;
+    for (uint32_t s = 0; s < L->slots.size(); ++s) {
+      if (req_of[s] < 0) continue;
+      const dsa::LlrState &st = L->host_state[s];
+      if (st.phase != dsa::kLlrDone) continue;
+      const uint32_t r = (uint32_t)req_of[s];
+      dsa_llr_status_get(L, s, &status[r]);
+      done_s[r] = now();
+      dsa_llr_remove(L, s);
+      req_of[s] = -1;
+      --active;
+      ++finished;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -82,6 +82,22 @@ _SIGS = {
     "dsa_chain_last_sweep_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "dsa_schedule_query": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dsa_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "dsa_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(ChainParams), C.POINTER(C.c_uint32)]),
+    "dsa_batch_finalize": (C.c_int, [C.c_void_p]),
+    "dsa_batch_destroy": (None, [C.c_void_p]),
+    "dsa_batch_sweep": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "dsa_batch_measure": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dsa_batch_run_cycles": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "dsa_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint32)]),
+    "dsa_batch_get_links": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "dsa_batch_counters": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "dsa_batch_last_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "dsa_chain_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dsa_chain_kernel_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "dsa_probe_gather": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
+    "dsa_probe_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "dsa_last_error": (C.c_char_p, []),
     "dsa_version": (C.c_char_p, []),
 }
@@ -271,6 +287,203 @@ class Chain:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().dsa_chain_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Many independent chains swept together (dsa_batch_*): the device-side
+    replacement of the reference's one-process-per-point scans."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().dsa_batch_create(C.byref(h)))
+        self._h = h
+        self.shapes = []
+        self._groups = []
+        self.final = False
+
+    def add(self, group: Group, D: int, Nt: int, Nx: int, beta0: float, beta1: float, beta2: float, seed: int,
+            start: str = "hot", rng: str = "philox", order: str = "colour") -> int:
+        p = ChainParams(D, Nt, Nx, beta0, beta1, beta2, seed, DSA_START[start], DSA_RNG[rng], DSA_ORDER[order])
+        idx = C.c_uint32()
+        _check(lib().dsa_batch_add(self._h, group._h, C.byref(p), C.byref(idx)))
+        self.shapes.append((D, Nt, Nx))
+        self._groups.append(group)
+        return idx.value
+
+    def finalize(self) -> "Batch":
+        _check(lib().dsa_batch_finalize(self._h))
+        self.final = True
+        return self
+
+    def sweep(self, n: int = 1) -> None:
+        _check(lib().dsa_batch_sweep(self._h, n))
+
+    update = sweep
+
+    def measure(self) -> np.ndarray:
+        out = np.empty((len(self.shapes), 4), np.float64)
+        _check(lib().dsa_batch_measure(self._h, _ptr(out)))
+        return out
+
+    def run_cycles(self, ncycles: int, sweeps_per_cycle: int) -> np.ndarray:
+        out = np.empty((ncycles, len(self.shapes), 4), np.float64)
+        _check(lib().dsa_batch_run_cycles(self._h, ncycles, sweeps_per_cycle, _ptr(out)))
+        return out
+
+    def links(self, chain: int) -> np.ndarray:
+        D, Nt, Nx = self.shapes[chain]
+        out = np.empty(Nt * Nx ** (D - 1) * D, np.int32)
+        _check(lib().dsa_batch_get_links(self._h, chain, _ptr(out)))
+        return out
+
+    def counters(self):
+        n = len(self.shapes)
+        hit = np.empty(n, np.uint64)
+        acc = np.empty(n, np.uint64)
+        _check(lib().dsa_batch_counters(self._h, _ptr(hit), _ptr(acc)))
+        return hit, acc
+
+    def info(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(lib().dsa_batch_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def last_ms(self) -> float:
+        ms = C.c_double()
+        _check(lib().dsa_batch_last_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().dsa_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class LlrParams(C.Structure):
+    _fields_ = [
+        ("D", C.c_uint32), ("Nt", C.c_uint32), ("Nx", C.c_uint32),
+        ("axis", C.c_int32), ("bg", C.c_double), ("Ewin", C.c_double), ("hw", C.c_double), ("a0", C.c_double),
+        ("seed", C.c_int32), ("start", C.c_int32), ("K", C.c_uint32), ("NRM", C.c_uint32),
+        ("tol", C.c_double), ("cap", C.c_uint32),
+    ]
+
+
+class LlrStatus(C.Structure):
+    _fields_ = [
+        ("a", C.c_double), ("S1", C.c_double), ("S2", C.c_double), ("last_da", C.c_double),
+        ("maxdrift", C.c_double), ("phase", C.c_uint32), ("iterations", C.c_uint32),
+        ("seed_sweeps", C.c_uint32), ("reached", C.c_uint32), ("sweeps", C.c_uint64), ("hit", C.c_uint64),
+        ("acc", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+LLR_PHASE = {0: "seed", 1: "rm", 2: "done", 3: "idle"}
+
+
+def llr_params(D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, start="hot", K=20, NRM=50, tol=0.0, cap=0):
+    return LlrParams(D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, DSA_START[start], K, NRM, tol, cap)
+
+
+class LlrEngine:
+    """Device-resident LLR chains (dsa_llr_*): llr.cpp's constrained update,
+    drive-in and Robbins-Monro iteration for many windows at once."""
+
+    def __init__(self, D: int = 4, max_chains: int = 64, hbm_budget: int = 0):
+        L = lib()
+        if not hasattr(L.dsa_llr_create, "_sig"):
+            vp = C.c_void_p
+            L.dsa_llr_create.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(vp)]
+            L.dsa_llr_destroy.argtypes = [vp]
+            L.dsa_llr_destroy.restype = None
+            L.dsa_llr_add.argtypes = [vp, vp, C.POINTER(LlrParams), vp, C.POINTER(C.c_uint32)]
+            L.dsa_llr_remove.argtypes = [vp, C.c_uint32]
+            L.dsa_llr_step.argtypes = [vp, C.c_uint32]
+            L.dsa_llr_status_get.argtypes = [vp, C.c_uint32, C.POINTER(LlrStatus)]
+            L.dsa_llr_get_links.argtypes = [vp, C.c_uint32, vp]
+            L.dsa_llr_footprint.argtypes = [C.POINTER(LlrParams), C.POINTER(C.c_uint64)]
+            L.dsa_llr_serve.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]
+            L.dsa_llr_create._sig = True
+        h = C.c_void_p()
+        _check(L.dsa_llr_create(D, max_chains, hbm_budget, C.byref(h)))
+        self._h = h
+        self.D = D
+        self._shapes = {}
+        self._keep = []
+
+    def add(self, group: Group, params: LlrParams, links=None) -> int:
+        slot = C.c_uint32()
+        ptr = None
+        if links is not None:
+            arr = np.ascontiguousarray(links, dtype=np.int32)
+            ptr = _ptr(arr)
+        _check(lib().dsa_llr_add(self._h, group._h, C.byref(params), ptr, C.byref(slot)))
+        self._shapes[slot.value] = (params.D, params.Nt, params.Nx)
+        self._keep.append(group)
+        return slot.value
+
+    def remove(self, slot: int) -> None:
+        _check(lib().dsa_llr_remove(self._h, slot))
+        self._shapes.pop(slot, None)
+
+    def step(self, n: int = 1) -> None:
+        _check(lib().dsa_llr_step(self._h, n))
+
+    def status(self, slot: int) -> dict:
+        s = LlrStatus()
+        _check(lib().dsa_llr_status_get(self._h, slot, C.byref(s)))
+        return s.as_dict()
+
+    def links(self, slot: int) -> np.ndarray:
+        D, Nt, Nx = self._shapes[slot]
+        out = np.empty(Nt * Nx ** (D - 1) * D, np.int32)
+        _check(lib().dsa_llr_get_links(self._h, slot, _ptr(out)))
+        return out
+
+    def serve(self, groups, params, arrivals, chunk: int = 5):
+        """Continuous batching over a request stream; returns (admit_s, done_s, statuses)."""
+        n = len(params)
+        garr = (C.c_void_p * n)(*[g._h.value for g in groups])
+        parr = (LlrParams * n)(*params)
+        arr = np.ascontiguousarray(arrivals, np.float64)
+        admit = np.zeros(n, np.float64)
+        done = np.zeros(n, np.float64)
+        st = (LlrStatus * n)()
+        self._keep.extend(groups)
+        _check(lib().dsa_llr_serve(self._h, C.cast(garr, C.c_void_p), C.cast(parr, C.c_void_p), _ptr(arr), n, chunk,
+                                   _ptr(admit), _ptr(done), C.cast(st, C.c_void_p)))
+        return admit, done, [s.as_dict() for s in st]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().dsa_llr_destroy(self._h)
             self._h = None
 
     def __del__(self):

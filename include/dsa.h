@@ -112,6 +112,104 @@ int dsa_schedule_query(uint32_t D, uint32_t Nt, uint32_t Nx, int32_t order, uint
 /* Milliseconds the last dsa_chain_sweep spent on the device (HIP events). */
 int dsa_chain_last_sweep_ms(dsa_chain *c, double *ms);
 
+/* ---- many chains at once -------------------------------------------------
+ * Replaces the reference's process-per-point scans (evermore.sh:1-23,
+ * jobs2run:1-55; each process runs dym-mod-metro.cpp:199-337): chains of
+ * different shapes (one D per batch), groups, couplings and seeds are swept
+ * together, one launch per pass index.  Each chain's links equal those of the
+ * same dsa_chain run alone.  Usage: create, add chains, finalize, then sweep /
+ * measure / run_cycles.  dsa_batch_add returns the chain's index. */
+typedef struct dsa_batch dsa_batch;
+int dsa_batch_create(dsa_batch **out);
+int dsa_batch_add(dsa_batch *b, const dsa_group *g, const dsa_chain_params *p, uint32_t *index);
+int dsa_batch_finalize(dsa_batch *b);
+void dsa_batch_destroy(dsa_batch *b);
+/* n sweeps of every chain (update(), dym-mod-metro.cpp:146-197). */
+int dsa_batch_sweep(dsa_batch *b, uint32_t n);
+/* out[c*4 + 0..3]: mean Re P, Im P, plaquette, 2x1 rectangle of chain c
+ * (dym-mod-metro.cpp:282-303), FP64 fixed-shape tree per chain. */
+int dsa_batch_measure(dsa_batch *b, double *out);
+/* ncycles x (sweeps_per_cycle sweeps + measure), captured once in a hipGraph
+ * and replayed; out[k*nchains*4 + c*4 + q] is cycle k's measurement. */
+int dsa_batch_run_cycles(dsa_batch *b, uint32_t ncycles, uint32_t sweeps_per_cycle, double *out);
+int dsa_batch_info(dsa_batch *b, uint32_t *nchains, uint32_t *npasses, uint32_t *nblocks_per_sweep);
+int dsa_batch_get_links(dsa_batch *b, uint32_t chain, int32_t *a);
+/* per-chain cumulative hit and acc counters (arrays of nchains) */
+int dsa_batch_counters(dsa_batch *b, uint64_t *hit, uint64_t *acc);
+int dsa_batch_last_ms(dsa_batch *b, double *ms);
+
+/* ---- LLR: energy-window constrained chains ---------------------------------
+ * Replaces update_constrained / seed_window / rm_solve of llr.cpp:178-250
+ * (llr2d.cpp:178-259 for two windows).  Each chain samples
+ * exp(-a E_w - bg E_bg) restricted to E_w in [Ewin - hw, Ewin + hw], E_w = S1
+ * (axis 1, plaquettes) or S2 (axis 2, rectangles); it first drives its
+ * energy into the window (greedy moves, at most `cap` sweeps), then runs
+ * Robbins-Monro iterations a += 12/(W^2 (m+1)) (<E_w> - Ewin), W = 2 hw, each
+ * over K constrained sweeps, until |delta a| < tol or NRM iterations.  The
+ * window is tested per colour pass (a pass leaving the window is undone) and
+ * energies are kept in 2^-32 fixed point.  Chains live in the slots of one
+ * engine (one D per engine) and are advanced together. */
+typedef struct {
+  uint32_t D, Nt, Nx;
+  int32_t axis;       /* 1: window S1 (llr.cpp AXIS=1), 2: window S2 */
+  double bg;          /* coupling of the other action (llr.cpp BG) */
+  double Ewin, hw;    /* window centre and half width */
+  double a0;          /* initial tilt */
+  int32_t seed;       /* Philox key of the chain */
+  int32_t start;      /* DSA_START_HOT | DSA_START_COLD (ignored when links are given) */
+  uint32_t K, NRM;    /* constrained sweeps per RM iteration; iteration cap */
+  double tol;         /* stop once |delta a| < tol; 0 runs NRM iterations */
+  uint32_t cap;       /* drive-in sweep cap (0: 20000, llr.cpp:225) */
+} dsa_llr_params;
+
+enum { DSA_LLR_SEED = 0, DSA_LLR_RM = 1, DSA_LLR_DONE = 2, DSA_LLR_IDLE = 3 };
+
+typedef struct {
+  double a;           /* current tilt a_n */
+  double S1, S2;      /* current plaquette / rectangle actions */
+  double last_da;     /* |delta a| of the last iteration */
+  double maxdrift;    /* max |incremental - recomputed| windowed energy */
+  uint32_t phase;     /* DSA_LLR_* */
+  uint32_t iterations, seed_sweeps, reached;
+  uint64_t sweeps, hit, acc;
+} dsa_llr_status;
+
+typedef struct dsa_llr dsa_llr;
+/* An engine of at most max_chains chains of dimension D; hbm_budget bounds the
+ * summed chain footprints (0: no bound). */
+int dsa_llr_create(uint32_t D, uint32_t max_chains, uint64_t hbm_budget, dsa_llr **out);
+void dsa_llr_destroy(dsa_llr *L);
+int dsa_llr_footprint(const dsa_llr_params *p, uint64_t *bytes);
+/* Admits a chain; init_links (reference order, may be NULL) seeds it from a
+ * given configuration, as run_pass does from the previous window (llr.cpp:256-274). */
+int dsa_llr_add(dsa_llr *L, const dsa_group *g, const dsa_llr_params *p, const int32_t *init_links, uint32_t *slot);
+int dsa_llr_remove(dsa_llr *L, uint32_t slot);
+/* nsweeps sweeps of every active chain (each in its own phase). */
+int dsa_llr_step(dsa_llr *L, uint32_t nsweeps);
+int dsa_llr_status_get(dsa_llr *L, uint32_t slot, dsa_llr_status *out);
+int dsa_llr_get_links(dsa_llr *L, uint32_t slot, int32_t *a);
+/* Continuous-batching service of nreq window requests arriving at arrival_s[r]
+ * seconds: admission by free slot and footprint, `chunk` sweeps per round for
+ * every active chain, retirement on convergence.  Outputs per request: admit
+ * and done times (seconds from the call) and the final status.  (The
+ * reference serves windows one process at a time: scripts/llr2d_parallel.sh.) */
+int dsa_llr_serve(dsa_llr *L, const dsa_group *const *groups, const dsa_llr_params *req, const double *arrival_s,
+                  uint32_t nreq, uint32_t chunk, double *admit_s, double *done_s, dsa_llr_status *status);
+
+/* Measurement helpers (no counterpart in the reference, which times with
+ * gettimeofday around update(), timer.cpp:11-27).  With profiling on, every
+ * pass launch is bracketed by HIP events on the chain's stream and
+ * dsa_chain_kernel_time returns the summed kernel milliseconds and launch
+ * count since profiling was switched on. */
+int dsa_chain_set_profiling(dsa_chain *c, int32_t on);
+int dsa_chain_kernel_time(dsa_chain *c, double *ms, uint64_t *launches);
+/* Random uint16 gathers from the group's P x P table, `blocks` x 256 lanes,
+ * 24 gathers in flight per lane, `rounds` rounds: the ceiling the sweep's
+ * table gathers are bound by. */
+int dsa_probe_gather(const dsa_group *g, uint32_t blocks, uint32_t rounds, double *gathers_per_s);
+/* A plain device-to-device copy kernel of `bytes`, averaged over `reps`. */
+int dsa_probe_copy(uint64_t bytes, uint32_t reps, double *ms_per_copy);
+
 const char *dsa_last_error(void);
 const char *dsa_version(void);
 

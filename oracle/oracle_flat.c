@@ -115,3 +115,28 @@ void ogp_exp_many(const double *x, double *y, size_t n) {
 uint64_t ogp_minstd_seed(long long s) { return og_minstd_seed(s); }
 double ogp_canonical(uint64_t *x) { return og_canonical(x); }
 int ogp_randint(uint64_t *x, int lo, int hi) { return og_randint(x, lo, hi); }
+
+/* ---- LLR ---- */
+#include "oracle_llr.h"
+void *ogp_llr_new(void *g, unsigned D, unsigned Nt, unsigned Nx, int axis, double bg, double Ewin, double hw,
+                  double a0, int seed, int cold, unsigned K, unsigned NRM, double tol, unsigned cap) {
+  ogl_dev *L = (ogl_dev *)calloc(1, sizeof *L);
+  if (ogl_dev_init(L, (og_group *)g, D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, cold, K, NRM, tol, cap)) {
+    free(L);
+    return NULL;
+  }
+  return L;
+}
+void ogp_llr_free(void *L) { if (L) { ogl_dev_free((ogl_dev *)L); free(L); } }
+void ogp_llr_step(void *L, unsigned n) { ogl_dev_step((ogl_dev *)L, n); }
+/* out: a, S1, S2, last_da, maxdrift, phase, m, seed_sweeps, reached, sweep, hit, acc */
+void ogp_llr_status(void *Lv, double *out) {
+  ogl_dev *L = (ogl_dev *)Lv;
+  out[0] = L->a; out[1] = (double)L->S1 / 4294967296.0; out[2] = (double)L->S2 / 4294967296.0;
+  out[3] = L->last_da; out[4] = L->maxdrift; out[5] = L->phase; out[6] = L->m; out[7] = L->seed_sweeps;
+  out[8] = L->reached; out[9] = (double)L->sweep; out[10] = (double)L->hit; out[11] = (double)L->acc;
+}
+void ogp_llr_links(void *Lv, int32_t *out) {
+  ogl_dev *L = (ogl_dev *)Lv;
+  memcpy(out, L->c.L.a, (size_t)L->c.L.V * L->c.L.D * 4);
+}

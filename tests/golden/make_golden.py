@@ -82,6 +82,34 @@ def cli_fixture(name, args, threads=1):
         f.write("\n".join(lines) + "\n")
 
 
+def replicas_fixture(name, group, head, seeds, tail, nx, nt):
+    """Per-replica means of the GMES/RMES observables over independent seeds."""
+    rows = []
+    for sd in seeds:
+        with tempfile.TemporaryDirectory() as td:
+            out = run([os.path.join(REFBIN, "dym-mod-metro"), os.path.join(REF, "groups", group)] + head +
+                      [str(sd)] + tail, cwd=td)
+        g = [l.split() for l in out.splitlines() if l.startswith("GMES:")]
+        r = [l.split() for l in out.splitlines() if l.startswith("RMES:")]
+        arr = np.array([[float(v) for v in row[1:]] for row in g])
+        rect = np.array([float(row[2]) for row in r])
+        rows.append(np.concatenate([[arr[:, 3].mean(), rect.mean(), np.hypot(arr[:, 1], arr[:, 2]).mean(),
+                                     arr[:, 1].mean()], arr[:, 4:].mean(0)]))
+    np.savez_compressed(os.path.join(HERE, f"replicas_{name}.npz"), group=group, args=np.array(head + tail),
+                        seeds=np.array(list(seeds)), nx=nx, nt=nt, means=np.array(rows),
+                        columns=np.array(["plaq", "rect", "absP", "ReP"] +
+                                         [f"W{k + 1}x{l + 1}" for k in range(nx) for l in range(nt)]))
+
+
+def llr_fixture(name, args):
+    """stdout of the reference llr driver (llr.cpp) at one thread."""
+    with tempfile.TemporaryDirectory() as td:
+        out = run([os.path.join(REFBIN, "llr"), os.path.join(REF, "groups", args[0])] + args[1:], cwd=td)
+    with open(os.path.join(HERE, f"llr_{name}.txt"), "w") as f:
+        f.write("# llr " + " ".join(args) + "  (OMP_NUM_THREADS=1)\n")
+        f.write(out)
+
+
 def main():
     if not os.path.exists(os.path.join(REFBIN, "ref_harness")):
         sys.exit("build the reference first: make -f oracle/ref.mk")
@@ -100,10 +128,23 @@ def main():
     traj_fixture("mybi_3x3", "myBI", 4, 3, 3, 0.0, 1.0, 0.5, 77, 4)
     # full CLI runs (config 1; statistics runs)
     cli_fixture("c1", [os.path.join(REF, "groups", g), "4", "2", "2", "0", "1.5", "0.1", "1", "10", "100", "100"])
-    cli_fixture("stats_c1", [os.path.join(REF, "groups", g), "4", "2", "2", "0", "1.5", "0.1", "7", "10", "2000",
-                             "100"])
-    cli_fixture("stats_4x4", [os.path.join(REF, "groups", g), "4", "4", "4", "0", "3.0", "-0.25", "9", "4", "500",
-                              "100"])
+    # long single chains where the Metropolis chain mixes (acceptance 0.2-0.4)
+    cli_fixture("stats_4x4", [os.path.join(REF, "groups", g), "4", "4", "4", "0", "1.2", "-0.1", "9", "2", "1000",
+                              "50"])
+    cli_fixture("stats_6x6", [os.path.join(REF, "groups", g), "4", "6", "6", "0", "1.6", "-0.25", "19", "1", "600",
+                              "50"])
+    # config 1 is frozen (acceptance 0.002): compare ensembles of independent replicas
+    replicas_fixture("c1", g, ["4", "2", "2", "0", "1.5", "0.1"], range(1000, 1064), ["10", "10", "100"], 2, 2)
+    # llr.cpp: ladders of windows on axis 1 and 2 (exact-output fixtures), and
+    # one window solved 16 times (8 repeats x hot/cold) for a statistical check
+    llr_fixture("axis1_3x3", [g, "4", "3", "3", "1", "-0.1", "-500", "-800", "150", "60", "-1.0", "5", "3", "2",
+                              "1"])
+    llr_fixture("axis2_4x3", [g, "4", "4", "3", "2", "1.2", "-300", "-500", "100", "50", "0.5", "6", "2", "2",
+                              "1"])
+    llr_fixture("s216_axis1", ["S216", "4", "3", "4", "1", "0.0", "-900", "-1000", "100", "60", "1.0", "8", "2",
+                               "3", "1"])
+    llr_fixture("stats_4x4", [g, "4", "4", "4", "1", "-0.1", "-1500", "-1500", "100", "75", "1.5", "20", "20",
+                              "30", "8"])
     meta = {
         "generator": "tests/golden/make_golden.py",
         "reference_build": "oracle/ref.mk (g++ -O3 -fopenmp -std=c++11, as the reference Makefile)",

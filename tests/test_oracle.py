@@ -317,3 +317,39 @@ def test_single_link_heat_bath_distribution(oracle, ogroups):
     # total-variation distance between the chain's histogram and the Boltzmann weights
     tv = 0.5 * np.abs(counts / counts.sum() - w).sum()
     assert tv < 0.06
+
+
+# ------------------------------------------------------------------ LLR
+
+@pytest.mark.parametrize("name", ["axis1_3x3", "axis2_4x3", "s216_axis1"])
+def test_llr_restatement_matches_reference_llr(name):
+    """oracle/oracle_llr (llr.cpp restated, one thread) prints exactly what the
+    reference llr binary printed (tests/golden/llr_*.txt)."""
+    import subprocess
+
+    path = os.path.join(GOLDEN, f"llr_{name}.txt")
+    lines = open(path).read().splitlines()
+    args = lines[0][2:].split("  (")[0].split()[1:]
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "build", "oracle_llr")
+    out = subprocess.run([exe, group_file(args[0])] + args[1:], capture_output=True, text=True, check=True).stdout
+    got = out.splitlines()
+    want = lines[1:]
+    # the first line echoes the group path
+    assert got[0].split(": ", 1)[1].split(" ", 1)[1] == want[0].split(": ", 1)[1].split(" ", 1)[1]
+    assert got[1:] == want[1:]
+
+
+def test_device_llr_restatement_keeps_energies_consistent(oracle, ogroups):
+    """The per-pass LLR of the device keeps its incremental fixed-point energies
+    equal to a recomputation, stays inside the window once reached, and moves."""
+    L = oracle.OLlr(ogroups("mys1080-v4"), 4, 4, 4, 1, -0.1, -1500.0, 75.0, 1.5, 3, K=5, NRM=4)
+    seen = set()
+    for _ in range(25):
+        L.step(1)
+        s = L.status()
+        if s["phase"] >= 1:
+            assert -1575.0 <= s["S1"] <= -1425.0
+        seen.add(round(s["S1"], 6))
+    assert s["phase"] == 2 and s["iterations"] == 4 and s["reached"] == 1
+    assert s["maxdrift"] < 1e-6
+    assert len(seen) > 10
