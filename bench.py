@@ -13,8 +13,8 @@ Prints one JSON line.  Besides the contract fields it carries
                 events on the chain's stream, and its random-gather ceiling
                 (608 table gathers per link-update vs a measured probe);
   cpu_baseline  the reference binary (oracle/_ref/dym-mod-metro, built from
-                the reference's sources) timed on this host's cores for one
-                sweep of the same workload, by its own TM_END: update line.
+                the reference's sources) timed on this host's cores for two
+                sweeps of the same workload, by its own TM_END: update line.
 """
 import argparse
 import gzip
@@ -91,12 +91,59 @@ def cpu_baseline(group_gz, cores):
         shutil.rmtree(td, ignore_errors=True)
 
 
+def scan_chains():
+    """The phase-diagram scan workload: 256 chains on a 16 x 16 grid of
+    beta1 in [1, 4] x beta2 in [-0.5, 0.3], shapes round-robin over 8^4, 12^4,
+    16^3 x 8, 8^3 x 16, seeds 100 + i, hot; chains 192..255 use S(216)."""
+    shapes = [(4, 8, 8), (4, 12, 12), (4, 8, 16), (4, 16, 8)]  # (D, Nt, Nx)
+    out = []
+    for i in range(256):
+        b1 = 1.0 + 3.0 * (i // 16) / 15.0
+        b2 = -0.5 + 0.8 * (i % 16) / 15.0
+        out.append(("S216" if i >= 192 else "mys1080-v4", shapes[i % 4], b1, b2, 100 + i))
+    return out
+
+
+def batched_vs_serial(dsa, sweeps):
+    """Link-updates/s of the 256-chain scan swept as one batch, and one chain
+    at a time (each chain's own launches), over the same number of sweeps."""
+    groups = {n: dsa.load_group(dsa.group_path(n)) for n in ("mys1080-v4", "S216")}
+    chains = scan_chains()
+    links = sum(s[1] * s[2] ** 3 * s[0] for _, s, *_ in chains)
+    b = dsa.Batch()
+    for name, (D, Nt, Nx), b1, b2, seed in chains:
+        b.add(groups[name], D, Nt, Nx, 0.0, b1, b2, seed)
+    b.finalize()
+    b.sweep(1)
+    t0 = time.perf_counter()
+    b.sweep(sweeps)
+    tb = time.perf_counter() - t0
+    b.close()
+    objs = []
+    for name, (D, Nt, Nx), b1, b2, seed in chains:
+        c = dsa.Chain(groups[name], D, Nt, Nx, 0.0, b1, b2, seed)
+        c.update(1)
+        objs.append(c)
+    t0 = time.perf_counter()
+    for c in objs:
+        c.update(sweeps)
+    ts = time.perf_counter() - t0
+    for c in objs:
+        c.close()
+    return {"workload": "256 chains, 16x16 (beta1, beta2) grid, shapes 8^4/12^4/16^3x8/8^3x16 round-robin, "
+                        "192 S(1080) + 64 S(216), hot seeds 100+i", "sweeps": sweeps, "links": links,
+            "batched_link_updates_per_s": sweeps * links / tb, "serial_link_updates_per_s": sweeps * links / ts,
+            "batched_s": tb, "serial_s": ts, "speedup": ts / tb}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scan-sweeps", type=int, default=10,
+                    help="sweeps of the 256-chain scan timed batched and serially (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy kernel of this size (scale for the FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -150,6 +197,11 @@ def main():
         if dsa.lib().dsa_probe_copy(args.copy_probe_mb << 20, 4, C.byref(cm)) == 0:
             copy_ms = cm.value
 
+    scan = None
+    if args.scan_sweeps:
+        log(f"bench: 256-chain scan, {args.scan_sweeps} sweeps batched and one chain at a time")
+        scan = batched_vs_serial(dsa, args.scan_sweeps)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -160,7 +212,7 @@ def main():
         cpu = cpu_baseline(gpath, cores)
 
     out = {
-        "metric": "Metropolis link-updates/s at 32^3x64 S1080",
+        "metric": "Metropolis link-updates/s at 32^3x64 S1080; batched-vs-serial chain speedup",
         "value": value,
         "unit": "link-updates/s",
         "n_gpus": 1,
@@ -186,6 +238,7 @@ def main():
                                 "frac": gather_rate / best if best else None,
                                 "per_unit": GATHERS_PER_LINK_UPDATE}},
         "cpu_baseline": cpu,
+        "batched_vs_serial": scan,
     }
     if copy_ms is not None:
         out["copy_probe"] = {"bytes": args.copy_probe_mb << 20, "ms": copy_ms}
