@@ -144,6 +144,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scan-sweeps", type=int, default=10,
                     help="sweeps of the 256-chain scan timed batched and serially (0: skip)")
+    ap.add_argument("--kernel", choices=["auto", "table", "class"], default="auto",
+                    help="sweep kernel (identical results): element values via L2 gathers, or class values in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy kernel of this size (scale for the FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -159,6 +161,7 @@ def main():
     g = dsa.load_group(gpath)
     log("bench: creating the 32^3x64 chain")
     c = dsa.Chain(g, D, NT, NX, *BETA, SEED, start="hot", rng="philox", order="colour")
+    c.set_kernel(args.kernel)
     npass, ncol = c.schedule_info()
     log(f"bench: {npass} passes per sweep, colours per direction {ncol}; warmup {args.warmup}")
     import ctypes as C

@@ -2,6 +2,7 @@
 // device memory, and the sweep / measurement drivers.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
+#include <cmath>
 #include <string.h>
 
 #include <map>
@@ -43,6 +44,7 @@ struct dsa_chain {
   uint32_t *d_sites = nullptr;
   uint32_t *d_mstate = nullptr;
   unsigned long long *d_acc = nullptr;
+  unsigned long long *d_fall = nullptr;  // re-decided hits (class kernel)
   int32_t *d_aos = nullptr;
   uint16_t *d_lines = nullptr;
   size_t lines_cap = 0;
@@ -55,6 +57,10 @@ struct dsa_chain {
   uint64_t sweep = 0, hit = 0;
   double last_ms = 0;
   int max_blocks = 2048;
+  // sweep kernel: DSA_KERNEL_*, whether the class kernel can run, its margin
+  int kernel = DSA_KERNEL_AUTO;
+  bool cls_ok = false;
+  double margin = 0, margin_scale = 1;
   // per-launch timing (measurement mode): one event pair per pass
   bool profiling = false;
   std::vector<hipEvent_t> pev;
@@ -72,7 +78,23 @@ dsa::Tables dsa::tables_view(const dsa_group *g, const DeviceTables &t) {
   T.P = g->g.P;
   T.npool = (uint32_t)g->g.pool.size();
   T.id = (uint32_t)g->g.id;
+  T.cls = t.cls;
+  T.cprod = t.cprod;
+  T.rcq = t.rcq;
+  T.ncls = (uint32_t)g->g.rc.size();
   return T;
+}
+
+double dsa::class_margin(const Group &g, int D, double beta0, double beta1, double beta2) {
+  const double u = 0x1p-53;
+  const int ns = 2 * (D - 1), nr = 6 * (D - 1);
+  auto gam = [u](int n) { return n * u / (1.0 - n * u); };
+  const double q = 0x1p-25;  // quantisation of the class values
+  const double e1 = ns * (g.dmax + q) + gam(ns) * ns * g.rmax;  // |r1_exact - r1_class|
+  const double e2 = nr * (g.dmax + q) + gam(nr) * nr * g.rmax;  // |m1_exact - m1_class|
+  const double a = std::fabs(beta0) + std::fabs(beta1) * (ns * g.rmax + e1) + std::fabs(beta2) * (nr * g.rmax + e2);
+  const double b = std::fabs(beta1) * e1 + std::fabs(beta2) * e2 + 8.0 * u * a;  // per action
+  return 4.0 * (2.0 * b);
 }
 
 int dsa::check_params(const dsa_chain_params *p) {
@@ -130,6 +152,16 @@ int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
   HIP_TRY(hipMemcpy(t.pool, G.pool.data(), G.pool.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(t.retr, G.retr.data(), P * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(t.imtr, G.imtr.data(), P * sizeof(double), hipMemcpyHostToDevice));
+  if (!G.cprod.empty()) {
+    const size_t nb = dsa::pad16(G.cprod.size());
+    HIP_TRY(hipMalloc(&t.cls, P));
+    HIP_TRY(hipMalloc(&t.cprod, nb));
+    HIP_TRY(hipMalloc(&t.rcq, G.rcq.size() * sizeof(int32_t)));
+    HIP_TRY(hipMemset(t.cprod, 0, nb));
+    HIP_TRY(hipMemcpy(t.cls, G.cls.data(), P, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.cprod, G.cprod.data(), G.cprod.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.rcq, G.rcq.data(), G.rcq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   g->dev[device] = t;
   out = t;
   return 0;
@@ -143,6 +175,9 @@ void free_tables(DeviceTables &t) {
   if (t.pool) (void)hipFree(t.pool);
   if (t.retr) (void)hipFree(t.retr);
   if (t.imtr) (void)hipFree(t.imtr);
+  if (t.cls) (void)hipFree(t.cls);
+  if (t.cprod) (void)hipFree(t.cprod);
+  if (t.rcq) (void)hipFree(t.rcq);
   t = DeviceTables();
 }
 
@@ -276,12 +311,21 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
   if (hipMalloc(&c->d_links, nl * sizeof(uint16_t)) != hipSuccess ||
       hipMalloc(&c->d_aos, nl * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->d_sites, c->S.sites.size() * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&c->d_acc, sizeof(unsigned long long)) != hipSuccess)
+      hipMalloc(&c->d_acc, sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->d_fall, sizeof(unsigned long long)) != hipSuccess)
     return bail(fail(DSA_E_DEVICE, "device allocation failed"));
   if (hipMemcpy(c->d_sites, c->S.sites.data(), c->S.sites.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
           hipSuccess ||
-      hipMemset(c->d_acc, 0, sizeof(unsigned long long)) != hipSuccess)
+      hipMemset(c->d_acc, 0, sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(c->d_fall, 0, sizeof(unsigned long long)) != hipSuccess)
     return bail(fail(DSA_E_DEVICE, "device upload failed"));
+  if (c->T.cprod) {
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+      lds_max = 0;
+    c->margin = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2);
+    c->cls_ok = dsa::sweep_cls_lds_bytes(c->T) <= (size_t)lds_max && c->margin < 1e-3;
+  }
 
   const bool cold = p->start == DSA_START_COLD;
   if (p->rng == DSA_RNG_MINSTD) {
@@ -310,7 +354,7 @@ void dsa_chain_destroy(dsa_chain *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void *ptr : {(void *)c->d_links, (void *)c->d_sites, (void *)c->d_mstate, (void *)c->d_acc,
-                    (void *)c->d_aos, (void *)c->d_lines, (void *)c->d_partials, (void *)c->d_out})
+                    (void *)c->d_fall, (void *)c->d_aos, (void *)c->d_lines, (void *)c->d_partials, (void *)c->d_out})
     if (ptr) (void)hipFree(ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -333,6 +377,8 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.beta1 = c->p.beta1;
   A.beta2 = c->p.beta2;
   A.seed = (uint32_t)c->p.seed;
+  A.cmargin = c->margin * c->margin_scale;
+  A.fall = c->d_fall;
   const size_t np = c->S.passes.size();
   if (c->profiling && c->pev.size() < 2 * np) {
     for (size_t q = c->pev.size(); q < 2 * np; ++q) {
@@ -350,6 +396,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
       A.mu = ps.mu;
       A.sites = c->d_sites + ps.offset;
       A.count = ps.count;
+      A.use_cls = c->kernel == DSA_KERNEL_CLASS || (c->kernel == DSA_KERNEL_AUTO && c->cls_ok && ps.count >= 2048);
       if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q], c->stream));
       HIP_TRY(dsa::launch_sweep(A, c->max_blocks, c->stream));
       if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q + 1], c->stream));
@@ -372,6 +419,37 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   if (n) c->lines_valid = false;
+  return 0;
+}
+
+int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_set_kernel: NULL chain");
+  if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS)
+    return fail(DSA_E_ARG, "dsa_chain_set_kernel: unknown kernel " + std::to_string(kernel));
+  if (!(margin_scale >= 1.0) || !std::isfinite(margin_scale))
+    return fail(DSA_E_ARG, "dsa_chain_set_kernel: margin_scale must be finite and >= 1");
+  if (kernel == DSA_KERNEL_CLASS && !c->T.cprod)
+    return fail(DSA_E_STATE, "dsa_chain_set_kernel: the group has no class tables (order > 8192 or > 255 classes)");
+  if (kernel == DSA_KERNEL_CLASS) {
+    int lds_max = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    if (dsa::sweep_cls_lds_bytes(c->T) > (size_t)lds_max)
+      return fail(DSA_E_RANGE, "dsa_chain_set_kernel: class tables need " +
+                                   std::to_string(dsa::sweep_cls_lds_bytes(c->T)) + " bytes of LDS, the device has " +
+                                   std::to_string(lds_max));
+  }
+  c->kernel = kernel;
+  c->margin_scale = margin_scale;
+  return 0;
+}
+
+int dsa_chain_exact_decisions(dsa_chain *c, uint64_t *n) {
+  if (!c || !n) return fail(DSA_E_ARG, "dsa_chain_exact_decisions: NULL argument");
+  if (set_device(c)) return DSA_E_DEVICE;
+  unsigned long long v = 0;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(&v, c->d_fall, sizeof v, hipMemcpyDeviceToHost));
+  *n = v;
   return 0;
 }
 

@@ -230,6 +230,132 @@ __device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const Ta
   return acc;
 }
 
+// Class-product form of update_link, with identical results.
+//
+// ReTr is a class function, so the trial term ReTr(b' s_k) with b' = b p_j
+// equals ReTr(p_j t_k) for t_k = s_k b (conjugate by b): the class of p_j t_k
+// is cprod[j][t_k], an LDS lookup instead of an L2 gather, and t_k changes
+// only when a hit is accepted (t_k <- t_k p_j).  Class sums are exact
+// integers of the class ReTr in units of 2^-24 (kClsScale).  They differ from
+// the reference's element-value sums by at most the table's within-class
+// spread plus quantisation and rounding, bounded by `margin` for the change
+// of action; a hit whose decision could differ within that bound is decided
+// again from element values exactly as update_link does, so the links and
+// counters are those of update_link.
+constexpr double kClsScale = 0x1p-24;
+
+struct ClsCtx {
+  const uint8_t *s_cprod;  // LDS, npool x P class indices
+  const uint8_t *s_cls;    // LDS, P
+  const int32_t *s_rcq;    // LDS, class ReTr in units of 2^-24
+  const double *retr;      // global, element ReTr for exact decisions
+  double margin;
+};
+
+template <int D>
+__device__ inline uint32_t update_link_cls(const LinkCtx &C, const ClsCtx &K, const Geom &G, const TabView &T,
+                                           int mu, uint32_t i, uint32_t &nfall) {
+  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1), NL = NS + NR, NW = NL / 2;
+  const Site<D> s = decode<D>(G, i);
+  const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
+  uint32_t b = C.links[lp];
+  const uint32_t b0 = b;
+  // t_k = s_k b, two uint16 element indices per register (plaquettes first)
+  uint32_t t[NW];
+  {
+    uint32_t st[NS], re[NR];
+    build_staples<D>(C.links, G, T, s, mu, st, re);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const uint32_t lo = 2 * k < NS ? st[2 * k] : re[2 * k - NS];
+      const uint32_t hi = 2 * k + 1 < NS ? st[2 * k + 1] : re[2 * k + 1 - NS];
+      t[k] = T.M(lo, b) | (T.M(hi, b) << 16);
+    }
+  }
+  // class sum of the plaquette (k < NS) and rectangle terms for row `row`
+  auto sums = [&](const uint8_t *tab, int32_t &r, int32_t &m) {
+    int32_t a = 0, c = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const int32_t vlo = K.s_rcq[tab[t[k] & 0xffffu]];
+      const int32_t vhi = K.s_rcq[tab[t[k] >> 16]];
+      if (2 * k < NS) a += vlo; else c += vlo;
+      if (2 * k + 1 < NS) a += vhi; else c += vhi;
+    }
+    r = a;
+    m = c;
+  };
+  int32_t r1, m1;
+  sums(K.s_cls, r1, m1);
+  uint32_t ms = 0;
+  if (C.rng == kRngMinstd) ms = C.mstate[i];
+  const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
+  const double b1q = C.beta1 * kClsScale, b2q = C.beta2 * kClsScale;
+  uint32_t acc = 0;
+  for (int h = 0; h < kNHit; ++h) {
+    double u1, u2;
+    if (C.rng == kRngPhilox) {
+      const U4 w = philox4x32_10(U4{link, (uint32_t)h, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
+      u1 = u53(w.x, w.y);
+      u2 = u53(w.z, w.w);
+    } else {
+      u1 = minstd_canonical(ms);
+      u2 = minstd_canonical(ms);
+    }
+    const uint32_t jp = (uint32_t)(C.npool_d * u1);
+    int32_t r1n, m1n;
+    sums(K.s_cprod + jp * T.P, r1n, m1n);
+    // class-valued change of action; the exact one lies within M of it
+    const double dc = b1q * ((double)r1n - (double)r1) + b2q * ((double)m1n - (double)m1);
+    const double M = K.margin + 0x1p-50 * fabs(dc);
+    int dec;
+    if (dc + M <= 0.0) {
+      dec = 1;
+    } else if (dc - M > 0.0) {
+      const double e = dsa_exp(-dc);
+      const double w = 1.02 * M + 0x1p-46;
+      dec = u2 < e * (1.0 - w) ? 1 : (u2 >= e * (1.0 + w) ? 0 : -1);
+    } else {
+      dec = u2 < 1.0 - (2.02 * M + 0x1p-46) ? 1 : -1;
+    }
+    const uint32_t p = C.s_pool[jp];
+    if (dec < 0) {
+      // element values as update_link sums them: s_k = t_k b^-1, so
+      // ReTr(b s_k) = retr[b t_k b^-1] and ReTr(b' s_k) = retr[b' t_k b^-1]
+      const uint32_t bn = T.M(b, p), bi = T.I(b);
+      double e1 = 0, e2 = 0, e1n = 0, e2n = 0;
+#pragma unroll
+      for (int k = 0; k < NL; ++k) {
+        const uint32_t tk = (k & 1) ? t[k / 2] >> 16 : t[k / 2] & 0xffffu;
+        const double v = K.retr[T.M(T.M(b, tk), bi)], vn = K.retr[T.M(T.M(bn, tk), bi)];
+        if (k < NS) {
+          e1 += v;
+          e1n += vn;
+        } else {
+          e2 += v;
+          e2n += vn;
+        }
+      }
+      const double oldact = C.beta0 + C.beta1 * e1 + C.beta2 * e2;
+      const double newact = C.beta0 + C.beta1 * e1n + C.beta2 * e2n;
+      const double probrat = (newact <= oldact) ? 1.0 : dsa_exp(-(newact - oldact));
+      dec = probrat > u2 ? 1 : 0;
+      ++nfall;
+    }
+    if (dec) {
+      b = T.M(b, p);
+#pragma unroll
+      for (int k = 0; k < NW; ++k) t[k] = T.M(t[k] & 0xffffu, p) | (T.M(t[k] >> 16, p) << 16);
+      r1 = r1n;
+      m1 = m1n;
+      ++acc;
+    }
+  }
+  if (b != b0) C.links[lp] = (uint16_t)b;
+  if (C.rng == kRngMinstd) C.mstate[i] = ms;
+  return acc;
+}
+
 // Loads ReTr, the inverse table and the proposal pool of one group into LDS.
 __device__ inline void load_tables_lds(const Tables &Tb, double *s_retr, uint16_t *s_inv, uint16_t *s_pool) {
   for (uint32_t t = threadIdx.x; t < Tb.P; t += blockDim.x) {

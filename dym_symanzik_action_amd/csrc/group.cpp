@@ -2,6 +2,9 @@
 // instead of abort()).
 #include "group.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 #include <stdlib.h>
 #include <string.h>
 #include <zlib.h>
@@ -94,7 +97,47 @@ std::string load_group(const char *path, Group &g) {
   g.min_retr = min_retr;
   g.nn_retr = nn_retr;
   if (g.pool.empty()) return "group has no proposal elements (all ReTr equal)";
+  build_classes(g);
   return std::string();
+}
+
+void build_classes(Group &g) {
+  g.cls.clear();
+  g.cprod.clear();
+  g.rc.clear();
+  g.rcq.clear();
+  g.dmax = g.rmax = 0;
+  const size_t P = g.P;
+  if (P == 0 || P > kMaxClassOrder) return;
+  std::vector<int> cls(P, -1);
+  int nc = 0;
+  for (size_t x = 0; x < P; ++x) {
+    if (cls[x] >= 0) continue;
+    if (nc == 255) return;
+    // the class of x: { y x y^-1 }
+    for (size_t y = 0; y < P; ++y) {
+      const size_t yx = g.mult[y * P + x];
+      cls[g.mult[yx * P + g.inv[y]]] = nc;
+    }
+    g.rc.push_back(g.retr[x]);
+    ++nc;
+  }
+  g.cls.resize(P);
+  for (size_t e = 0; e < P; ++e) {
+    g.cls[e] = (uint8_t)cls[e];
+    g.dmax = std::max(g.dmax, std::fabs(g.retr[e] - g.rc[cls[e]]));
+    g.rmax = std::max(g.rmax, std::fabs(g.retr[e]));
+  }
+  if (g.rmax > 4.0) {  // class sums must fit int32 in units of 2^-24
+    g.cls.clear();
+    g.rc.clear();
+    g.dmax = g.rmax = 0;
+    return;
+  }
+  for (double v : g.rc) g.rcq.push_back((int32_t)std::llround(std::ldexp(v, 24)));
+  g.cprod.resize(g.pool.size() * P);
+  for (size_t j = 0; j < g.pool.size(); ++j)
+    for (size_t t = 0; t < P; ++t) g.cprod[j * P + t] = g.cls[g.mult[(size_t)g.pool[j] * P + t]];
 }
 
 }  // namespace dsa

@@ -19,7 +19,19 @@ struct Group {
   std::vector<uint16_t> inv;       // P
   std::vector<uint16_t> pool;      // proposal elements (nearest the identity)
   double min_retr = 0, nn_retr = 0;
+  // Conjugacy classes, for the class-product sweep kernel (empty when the
+  // group has more than 255 classes or P > kMaxClassOrder).  rc[c] is the
+  // ReTr of the lowest-index element of class c; dmax = max |retr[e] -
+  // rc[cls[e]]| (the table's ReTr is a class function up to rounding), rmax =
+  // max |retr|.  cprod[j*P + t] = cls[pool[j] * t]; rcq[c] = rc[c] in units
+  // of 2^-24, rounded to nearest.
+  std::vector<uint8_t> cls, cprod;
+  std::vector<double> rc;
+  std::vector<int32_t> rcq;
+  double dmax = 0, rmax = 0;
 };
+
+constexpr unsigned kMaxClassOrder = 8192;
 
 // Parses a groups/ table file (plain text or gzip): P, P ReTr, P ImTr, P*P
 // table; trailing tokens are ignored (group.c:13-47).  The identity is the
@@ -29,5 +41,8 @@ struct Group {
 // (dym-mod-metro.cpp:239-245).  Returns an empty string on success, else a
 // message; never aborts.
 std::string load_group(const char *path, Group &g);
+
+// Fills cls, cprod, rc, dmax, rmax (left empty when not applicable).
+void build_classes(Group &g);
 
 }  // namespace dsa

@@ -18,6 +18,8 @@ int fail(int code, const std::string &msg);
 struct DeviceTables {
   uint16_t *mult = nullptr, *inv = nullptr, *pool = nullptr;
   double *retr = nullptr, *imtr = nullptr;
+  uint8_t *cls = nullptr, *cprod = nullptr;  // class tables (null if the group has none)
+  int32_t *rcq = nullptr;
 };
 
 }  // namespace dsa
@@ -33,6 +35,11 @@ namespace dsa {
 // The group's tables on `device`, uploaded on first use.
 int upload_tables(dsa_group *g, int device, DeviceTables &out);
 Tables tables_view(const dsa_group *g, const DeviceTables &t);
+// Bound on |dS_exact - dS_class| for one hit of the class-product kernel:
+// both trial actions' class-value error (D-dependent staple counts, the
+// table's within-class ReTr spread dmax) and the float rounding of both sums
+// and of beta0 + beta1 r1 + beta2 m1, with a safety factor of 4.
+double class_margin(const Group &g, int D, double beta0, double beta1, double beta2);
 // Shape and option checks shared by chains and batches.
 int check_params(const dsa_chain_params *p);
 // minstd-compat start (dym-mod-metro.cpp:227-228, 253-255): per-site states and

@@ -26,6 +26,12 @@ struct Tables {
   const double *imtr;    // P
   const uint16_t *pool;  // npool
   uint32_t P, npool, id;
+  // conjugacy classes (null when the group has none usable): class of each
+  // element, cprod[j*P + t] = class(pool[j] * t) padded to 16 bytes, and one
+  // ReTr per class in units of 2^-24
+  const uint8_t *cls, *cprod;
+  const int32_t *rcq;
+  uint32_t ncls;
 };
 
 enum RngKind : int { kRngMinstd = 0, kRngPhilox = 1 };
@@ -43,9 +49,17 @@ struct SweepArgs {
   double beta0, beta1, beta2;
   uint32_t seed;
   uint32_t sweep_lo, sweep_hi;
+  // class-product kernel: use it, the bound on |exact - class| trial-action
+  // differences below which a decision is recomputed exactly, and the count of
+  // such recomputations
+  int use_cls;
+  double cmargin;
+  unsigned long long *fall;
 };
 
 size_t sweep_lds_bytes(const Tables &T);
+size_t sweep_cls_lds_bytes(const Tables &T);
+__host__ __device__ inline size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s);
 
 // ---- many independent chains in one launch per pass ----

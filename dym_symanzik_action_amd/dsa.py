@@ -22,6 +22,7 @@ DSA_RNG = {"philox": 0, "minstd": 1}
 DSA_ORDER = {"colour": 0, "reference": 1}
 DSA_START = {"hot": 0, "cold": 1}
 DSA_SUM = {"tree": 0, "ordered": 1}
+DSA_KERNEL = {"auto": 0, "table": 1, "class": 2}
 LOOP_KIND = {"plaq": 0, "rect": 1, "poly": 2, "wilson": 3}
 
 
@@ -95,6 +96,8 @@ _SIGS = {
     "dsa_batch_counters": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "dsa_batch_last_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "dsa_chain_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dsa_chain_set_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "dsa_chain_exact_decisions": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "dsa_chain_kernel_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "dsa_probe_gather": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     "dsa_probe_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
@@ -224,6 +227,17 @@ class Chain:
     @sweeps_done.setter
     def sweeps_done(self, v: int) -> None:
         _check(lib().dsa_chain_set_sweep(self._h, v))
+
+    def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
+        """Sweep kernel: "table" (element ReTr via L2 gathers), "class" (class
+        values in LDS, exact re-decision near the bound), or "auto"; both give
+        identical links.  A huge margin_scale re-decides every hit exactly."""
+        _check(lib().dsa_chain_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
+
+    def exact_decisions(self) -> int:
+        n = C.c_uint64()
+        _check(lib().dsa_chain_exact_decisions(self._h, C.byref(n)))
+        return n.value
 
     def last_sweep_ms(self) -> float:
         ms = C.c_double()
