@@ -144,7 +144,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scan-sweeps", type=int, default=10,
                     help="sweeps of the 256-chain scan timed batched and serially (0: skip)")
-    ap.add_argument("--kernel", choices=["auto", "table", "class"], default="auto",
+    ap.add_argument("--kernel", choices=["auto", "table", "class", "class-narrow"], default="auto",
                     help="sweep kernel (identical results): element values via L2 gathers, or class values in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy kernel of this size (scale for the FETCH/WRITE_SIZE counters)")

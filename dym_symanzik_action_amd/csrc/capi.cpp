@@ -57,10 +57,12 @@ struct dsa_chain {
   uint64_t sweep = 0, hit = 0;
   double last_ms = 0;
   int max_blocks = 2048;
-  // sweep kernel: DSA_KERNEL_*, whether the class kernel can run, its margin
+  // sweep kernel: DSA_KERNEL_*, the class-kernel forms that can run here
+  // (dsa::kCls*, kClsNone if neither), their decision margins
   int kernel = DSA_KERNEL_AUTO;
-  bool cls_ok = false;
-  double margin = 0, margin_scale = 1;
+  int cls_best = dsa::kClsNone;
+  bool narrow_ok = false;
+  double margin8 = 0, margin16 = 0, margin_scale = 1;
   // per-launch timing (measurement mode): one event pair per pass
   bool profiling = false;
   std::vector<hipEvent_t> pev;
@@ -82,19 +84,24 @@ dsa::Tables dsa::tables_view(const dsa_group *g, const DeviceTables &t) {
   T.cprod = t.cprod;
   T.rcq = t.rcq;
   T.ncls = (uint32_t)g->g.rc.size();
+  T.cval = t.cval;
+  T.cval0 = t.cval0;
   return T;
 }
 
-double dsa::class_margin(const Group &g, int D, double beta0, double beta1, double beta2) {
+double dsa::class_margin(const Group &g, int D, double beta0, double beta1, double beta2, double q) {
   const double u = 0x1p-53;
   const int ns = 2 * (D - 1), nr = 6 * (D - 1);
   auto gam = [u](int n) { return n * u / (1.0 - n * u); };
-  const double q = 0x1p-25;  // quantisation of the class values
-  const double e1 = ns * (g.dmax + q) + gam(ns) * ns * g.rmax;  // |r1_exact - r1_class|
-  const double e2 = nr * (g.dmax + q) + gam(nr) * nr * g.rmax;  // |m1_exact - m1_class|
+  // |r1_exact - r1_class| <= e1 + f1: value differences (exact bounds) plus
+  // the rounding of the reference's float sum; likewise m1
+  const double e1 = ns * (g.dmax + q), f1 = gam(ns) * ns * g.rmax;
+  const double e2 = nr * (g.dmax + q), f2 = gam(nr) * nr * g.rmax;
   const double a = std::fabs(beta0) + std::fabs(beta1) * (ns * g.rmax + e1) + std::fabs(beta2) * (nr * g.rmax + e2);
-  const double b = std::fabs(beta1) * e1 + std::fabs(beta2) * e2 + 8.0 * u * a;  // per action
-  return 4.0 * (2.0 * b);
+  const double vals = std::fabs(beta1) * e1 + std::fabs(beta2) * e2;                   // per action
+  const double rounding = std::fabs(beta1) * f1 + std::fabs(beta2) * f2 + 8.0 * u * a;  // per action
+  // both actions of a hit; the rounding terms with a safety factor of 4
+  return 2.0 * vals * (1.0 + 1e-6) + 4.0 * (2.0 * rounding);
 }
 
 int dsa::check_params(const dsa_chain_params *p) {
@@ -162,6 +169,14 @@ int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
     HIP_TRY(hipMemcpy(t.cprod, G.cprod.data(), G.cprod.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.rcq, G.rcq.data(), G.rcq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   }
+  if (!G.cval.empty()) {
+    const size_t nb = dsa::pad16(G.cval.size() * sizeof(int16_t));
+    HIP_TRY(hipMalloc(&t.cval, nb));
+    HIP_TRY(hipMalloc(&t.cval0, P * sizeof(int16_t)));
+    HIP_TRY(hipMemset(t.cval, 0, nb));
+    HIP_TRY(hipMemcpy(t.cval, G.cval.data(), G.cval.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.cval0, G.cval0.data(), P * sizeof(int16_t), hipMemcpyHostToDevice));
+  }
   g->dev[device] = t;
   out = t;
   return 0;
@@ -178,6 +193,8 @@ void free_tables(DeviceTables &t) {
   if (t.cls) (void)hipFree(t.cls);
   if (t.cprod) (void)hipFree(t.cprod);
   if (t.rcq) (void)hipFree(t.rcq);
+  if (t.cval) (void)hipFree(t.cval);
+  if (t.cval0) (void)hipFree(t.cval0);
   t = DeviceTables();
 }
 
@@ -323,8 +340,15 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
       lds_max = 0;
-    c->margin = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2);
-    c->cls_ok = dsa::sweep_cls_lds_bytes(c->T) <= (size_t)lds_max && c->margin < 1e-3;
+    // quantisation: half a unit of the class values (2^-24 narrow, 2^-13 wide)
+    c->margin8 = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2, 0x1p-25);
+    c->margin16 = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2, 0x1p-14);
+    c->narrow_ok = dsa::sweep_cls_lds_bytes(c->T, dsa::kClsNarrow) <= (size_t)lds_max;
+    const bool wide_ok = c->T.cval && dsa::sweep_cls_lds_bytes(c->T, dsa::kClsWide) <= (size_t)lds_max;
+    if (wide_ok && c->margin16 < 1e-2)
+      c->cls_best = dsa::kClsWide;
+    else if (c->narrow_ok && c->margin8 < 1e-3)
+      c->cls_best = dsa::kClsNarrow;
   }
 
   const bool cold = p->start == DSA_START_COLD;
@@ -377,8 +401,13 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.beta1 = c->p.beta1;
   A.beta2 = c->p.beta2;
   A.seed = (uint32_t)c->p.seed;
-  A.cmargin = c->margin * c->margin_scale;
   A.fall = c->d_fall;
+  // the class form this chain runs for passes large enough to fill the device
+  int mode = dsa::kClsNone;
+  if (c->kernel == DSA_KERNEL_CLASS) mode = c->cls_best;
+  if (c->kernel == DSA_KERNEL_CLASS_NARROW) mode = dsa::kClsNarrow;
+  if (c->kernel == DSA_KERNEL_AUTO) mode = c->cls_best;
+  A.cmargin = (mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
   const size_t np = c->S.passes.size();
   if (c->profiling && c->pev.size() < 2 * np) {
     for (size_t q = c->pev.size(); q < 2 * np; ++q) {
@@ -396,7 +425,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
       A.mu = ps.mu;
       A.sites = c->d_sites + ps.offset;
       A.count = ps.count;
-      A.use_cls = c->kernel == DSA_KERNEL_CLASS || (c->kernel == DSA_KERNEL_AUTO && c->cls_ok && ps.count >= 2048);
+      A.use_cls = (c->kernel != DSA_KERNEL_AUTO || ps.count >= 2048) ? mode : dsa::kClsNone;
       if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q], c->stream));
       HIP_TRY(dsa::launch_sweep(A, c->max_blocks, c->stream));
       if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q + 1], c->stream));
@@ -424,20 +453,16 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
 
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale) {
   if (!c) return fail(DSA_E_ARG, "dsa_chain_set_kernel: NULL chain");
-  if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS)
+  if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS &&
+      kernel != DSA_KERNEL_CLASS_NARROW)
     return fail(DSA_E_ARG, "dsa_chain_set_kernel: unknown kernel " + std::to_string(kernel));
   if (!(margin_scale >= 1.0) || !std::isfinite(margin_scale))
     return fail(DSA_E_ARG, "dsa_chain_set_kernel: margin_scale must be finite and >= 1");
-  if (kernel == DSA_KERNEL_CLASS && !c->T.cprod)
-    return fail(DSA_E_STATE, "dsa_chain_set_kernel: the group has no class tables (order > 8192 or > 255 classes)");
-  if (kernel == DSA_KERNEL_CLASS) {
-    int lds_max = 0;
-    HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-    if (dsa::sweep_cls_lds_bytes(c->T) > (size_t)lds_max)
-      return fail(DSA_E_RANGE, "dsa_chain_set_kernel: class tables need " +
-                                   std::to_string(dsa::sweep_cls_lds_bytes(c->T)) + " bytes of LDS, the device has " +
-                                   std::to_string(lds_max));
-  }
+  if (kernel == DSA_KERNEL_CLASS && c->cls_best == dsa::kClsNone)
+    return fail(DSA_E_STATE, "dsa_chain_set_kernel: no class kernel for this group on this device (order > 8192, "
+                             "more than 63 classes, or class tables beyond the LDS)");
+  if (kernel == DSA_KERNEL_CLASS_NARROW && !c->narrow_ok)
+    return fail(DSA_E_STATE, "dsa_chain_set_kernel: the narrow class tables do not fit this device's LDS");
   c->kernel = kernel;
   c->margin_scale = margin_scale;
   return 0;

@@ -234,63 +234,75 @@ __device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const Ta
 //
 // ReTr is a class function, so the trial term ReTr(b' s_k) with b' = b p_j
 // equals ReTr(p_j t_k) for t_k = s_k b (conjugate by b): the class of p_j t_k
-// is cprod[j][t_k], an LDS lookup instead of an L2 gather, and t_k changes
-// only when a hit is accepted (t_k <- t_k p_j).  Class sums are exact
-// integers of the class ReTr in units of 2^-24 (kClsScale).  They differ from
-// the reference's element-value sums by at most the table's within-class
-// spread plus quantisation and rounding, bounded by `margin` for the change
-// of action; a hit whose decision could differ within that bound is decided
-// again from element values exactly as update_link does, so the links and
-// counters are those of update_link.
-constexpr double kClsScale = 0x1p-24;
+// is looked up in an LDS table instead of an L2 gather, and t_k changes only
+// when a hit is accepted (t_k <- t_k p_j).  Class sums are exact integers of
+// quantised class values (ClsNarrow: 2^-24 units through a class index;
+// ClsWide: 2^-13 units held in the table itself, one lookup per term).  They
+// differ from the reference's element-value sums by at most the table's
+// within-class spread plus quantisation and rounding, bounded by `margin` for
+// the change of action; a hit whose decision could differ within that bound
+// is decided again from element values exactly as update_link does, so the
+// links and counters are those of update_link.
+struct ClsNarrow {
+  const uint8_t *s_cprod;  // LDS, npool x P: 4 x class of pool[j] * t
+  const uint8_t *s_cls;    // LDS, P: 4 x class
+  const int32_t *s_rcq;    // LDS (static array), class ReTr in units of 2^-24
+  static constexpr double kScale = 0x1p-24;
+  using Row = const uint8_t *;
+  __device__ Row row(uint32_t jp, uint32_t P) const { return s_cprod + jp * P; }
+  __device__ Row row0() const { return s_cls; }
+  __device__ int32_t val(Row r, uint32_t t) const {
+    return *reinterpret_cast<const int32_t *>(reinterpret_cast<const uint8_t *>(s_rcq) + r[t]);
+  }
+};
+
+struct ClsWide {
+  const int16_t *s_cval;   // LDS, npool x P: class value of pool[j] * t in units of 2^-13
+  const int16_t *s_cval0;  // LDS, P: class value of t
+  static constexpr double kScale = 0x1p-13;
+  using Row = const int16_t *;
+  __device__ Row row(uint32_t jp, uint32_t P) const { return s_cval + jp * P; }
+  __device__ Row row0() const { return s_cval0; }
+  __device__ int32_t val(Row r, uint32_t t) const { return r[t]; }
+};
 
 struct ClsCtx {
-  const uint8_t *s_cprod;  // LDS, npool x P class indices
-  const uint8_t *s_cls;    // LDS, P
-  const int32_t *s_rcq;    // LDS, class ReTr in units of 2^-24
-  const double *retr;      // global, element ReTr for exact decisions
+  const double *retr;  // global, element ReTr for exact decisions
   double margin;
 };
 
-template <int D>
-__device__ inline uint32_t update_link_cls(const LinkCtx &C, const ClsCtx &K, const Geom &G, const TabView &T,
-                                           int mu, uint32_t i, uint32_t &nfall) {
-  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1), NL = NS + NR, NW = NL / 2;
+template <int D, class L>
+__device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const ClsCtx &K, const Geom &G,
+                                           const TabView &T, int mu, uint32_t i, uint32_t &nfall) {
+  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1), NL = NS + NR;
   const Site<D> s = decode<D>(G, i);
   const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
   uint32_t b = C.links[lp];
   const uint32_t b0 = b;
-  // t_k = s_k b, two uint16 element indices per register (plaquettes first)
-  uint32_t t[NW];
+  uint32_t t[NL];  // t_k = s_k b (plaquettes first)
   {
     uint32_t st[NS], re[NR];
     build_staples<D>(C.links, G, T, s, mu, st, re);
 #pragma unroll
-    for (int k = 0; k < NW; ++k) {
-      const uint32_t lo = 2 * k < NS ? st[2 * k] : re[2 * k - NS];
-      const uint32_t hi = 2 * k + 1 < NS ? st[2 * k + 1] : re[2 * k + 1 - NS];
-      t[k] = T.M(lo, b) | (T.M(hi, b) << 16);
-    }
+    for (int k = 0; k < NL; ++k) t[k] = T.M(k < NS ? st[k] : re[k - NS], b);
   }
-  // class sum of the plaquette (k < NS) and rectangle terms for row `row`
-  auto sums = [&](const uint8_t *tab, int32_t &r, int32_t &m) {
+  // class sums of the plaquette (k < NS) and rectangle terms through `row`
+  auto sums = [&](typename L::Row row, int32_t &r, int32_t &m) {
     int32_t a = 0, c = 0;
 #pragma unroll
-    for (int k = 0; k < NW; ++k) {
-      const int32_t vlo = K.s_rcq[tab[t[k] & 0xffffu]];
-      const int32_t vhi = K.s_rcq[tab[t[k] >> 16]];
-      if (2 * k < NS) a += vlo; else c += vlo;
-      if (2 * k + 1 < NS) a += vhi; else c += vhi;
+    for (int k = 0; k < NL; ++k) {
+      const int32_t v = Lk.val(row, t[k]);
+      if (k < NS) a += v; else c += v;
     }
     r = a;
     m = c;
   };
   int32_t r1, m1;
-  sums(K.s_cls, r1, m1);
+  sums(Lk.row0(), r1, m1);
   uint32_t ms = 0;
   if (C.rng == kRngMinstd) ms = C.mstate[i];
   const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
-  const double b1q = C.beta1 * kClsScale, b2q = C.beta2 * kClsScale;
+  const double b1q = C.beta1 * L::kScale, b2q = C.beta2 * L::kScale, margin = K.margin;
   uint32_t acc = 0;
   for (int h = 0; h < kNHit; ++h) {
     double u1, u2;
@@ -304,19 +316,26 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const ClsCtx &K, co
     }
     const uint32_t jp = (uint32_t)(C.npool_d * u1);
     int32_t r1n, m1n;
-    sums(K.s_cprod + jp * T.P, r1n, m1n);
+    sums(Lk.row(jp, T.P), r1n, m1n);
     // class-valued change of action; the exact one lies within M of it
     const double dc = b1q * ((double)r1n - (double)r1) + b2q * ((double)m1n - (double)m1);
-    const double M = K.margin + 0x1p-50 * fabs(dc);
-    int dec;
-    if (dc + M <= 0.0) {
-      dec = 1;
-    } else if (dc - M > 0.0) {
-      const double e = dsa_exp(-dc);
-      const double w = 1.02 * M + 0x1p-46;
-      dec = u2 < e * (1.0 - w) ? 1 : (u2 >= e * (1.0 + w) ? 0 : -1);
-    } else {
-      dec = u2 < 1.0 - (2.02 * M + 0x1p-46) ? 1 : -1;
+    const double M = margin + 0x1p-50 * fabs(dc);
+    // The decision on class values, certified against the exact one, without
+    // branches: 1 accept, 0 reject, -1 undecided.  Above the band (dc > M):
+    // v_exp_f32 (2^x, ~1 ulp) on the float-rounded argument is within 5e-6
+    // relative of exp(-dc) for dc < 60, so decide with 1.5e-5 of slack; and
+    // exp(-dS) < exp(-37) < 1e-15 <= u2 rejects.  Inside the band (|dc| <= M)
+    // exp(-dS) >= 1 - 2M accepts any smaller u2.
+    const double e = (double)__builtin_amdgcn_exp2f((float)(-fmin(fmax(dc, 0.0), 60.0) * 1.4426950408889634));
+    const double w = 1.02 * M + 0x1p-16;
+    const int above = u2 < e * (1.0 - w) ? 1 : (u2 >= e * (1.0 + w) ? 0 : -1);
+    int dec = dc + M <= 0.0  ? 1
+              : dc - M > 0.0 ? ((dc - M > 37.0 && u2 >= 1e-15) ? 0 : (dc < 60.0 ? above : -1))
+                             : (u2 < 1.0 - (2.02 * M + 0x1p-46) ? 1 : -1);
+    if (dec < 0 && dc - M > 0.0) {  // rare: decide with the double exp
+      const double ed = dsa_exp(-dc);
+      const double wd = 1.02 * M + 0x1p-46;
+      dec = u2 < ed * (1.0 - wd) ? 1 : (u2 >= ed * (1.0 + wd) ? 0 : -1);
     }
     const uint32_t p = C.s_pool[jp];
     if (dec < 0) {
@@ -326,7 +345,7 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const ClsCtx &K, co
       double e1 = 0, e2 = 0, e1n = 0, e2n = 0;
 #pragma unroll
       for (int k = 0; k < NL; ++k) {
-        const uint32_t tk = (k & 1) ? t[k / 2] >> 16 : t[k / 2] & 0xffffu;
+        const uint32_t tk = t[k];
         const double v = K.retr[T.M(T.M(b, tk), bi)], vn = K.retr[T.M(T.M(bn, tk), bi)];
         if (k < NS) {
           e1 += v;
@@ -345,7 +364,7 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const ClsCtx &K, co
     if (dec) {
       b = T.M(b, p);
 #pragma unroll
-      for (int k = 0; k < NW; ++k) t[k] = T.M(t[k] & 0xffffu, p) | (T.M(t[k] >> 16, p) << 16);
+      for (int k = 0; k < NL; ++k) t[k] = T.M(t[k], p);
       r1 = r1n;
       m1 = m1n;
       ++acc;

@@ -106,6 +106,8 @@ void build_classes(Group &g) {
   g.cprod.clear();
   g.rc.clear();
   g.rcq.clear();
+  g.cval.clear();
+  g.cval0.clear();
   g.dmax = g.rmax = 0;
   const size_t P = g.P;
   if (P == 0 || P > kMaxClassOrder) return;
@@ -113,7 +115,10 @@ void build_classes(Group &g) {
   int nc = 0;
   for (size_t x = 0; x < P; ++x) {
     if (cls[x] >= 0) continue;
-    if (nc == 255) return;
+    if (nc == (int)kMaxClasses) {
+      g.rc.clear();
+      return;
+    }
     // the class of x: { y x y^-1 }
     for (size_t y = 0; y < P; ++y) {
       const size_t yx = g.mult[y * P + x];
@@ -124,7 +129,7 @@ void build_classes(Group &g) {
   }
   g.cls.resize(P);
   for (size_t e = 0; e < P; ++e) {
-    g.cls[e] = (uint8_t)cls[e];
+    g.cls[e] = (uint8_t)(4 * cls[e]);
     g.dmax = std::max(g.dmax, std::fabs(g.retr[e] - g.rc[cls[e]]));
     g.rmax = std::max(g.rmax, std::fabs(g.retr[e]));
   }
@@ -138,6 +143,14 @@ void build_classes(Group &g) {
   g.cprod.resize(g.pool.size() * P);
   for (size_t j = 0; j < g.pool.size(); ++j)
     for (size_t t = 0; t < P; ++t) g.cprod[j * P + t] = g.cls[g.mult[(size_t)g.pool[j] * P + t]];
+  if (g.rmax < 3.999) {
+    std::vector<int16_t> v16;
+    for (double v : g.rc) v16.push_back((int16_t)std::lround(std::ldexp(v, 13)));
+    g.cval0.resize(P);
+    for (size_t t = 0; t < P; ++t) g.cval0[t] = v16[g.cls[t] / 4];
+    g.cval.resize(g.cprod.size());
+    for (size_t q = 0; q < g.cprod.size(); ++q) g.cval[q] = v16[g.cprod[q] / 4];
+  }
 }
 
 }  // namespace dsa

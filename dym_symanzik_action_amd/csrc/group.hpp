@@ -20,18 +20,23 @@ struct Group {
   std::vector<uint16_t> pool;      // proposal elements (nearest the identity)
   double min_retr = 0, nn_retr = 0;
   // Conjugacy classes, for the class-product sweep kernel (empty when the
-  // group has more than 255 classes or P > kMaxClassOrder).  rc[c] is the
-  // ReTr of the lowest-index element of class c; dmax = max |retr[e] -
-  // rc[cls[e]]| (the table's ReTr is a class function up to rounding), rmax =
-  // max |retr|.  cprod[j*P + t] = cls[pool[j] * t]; rcq[c] = rc[c] in units
-  // of 2^-24, rounded to nearest.
+  // group has more than kMaxClasses classes or P > kMaxClassOrder).  rc[c]
+  // is the ReTr of the lowest-index element of class c; dmax = max |retr[e] -
+  // rc[class(e)]| (the table's ReTr is a class function up to rounding), rmax
+  // = max |retr|; rcq[c] = rc[c] in units of 2^-24, rounded to nearest.
+  // cls[e] = 4 * class(e), the byte offset of e's value in an int32 array
+  // rcq, and cprod[j*P + t] = cls[pool[j] * t].  The wide form holds the
+  // values themselves: cval[j*P + t] = rc[class(pool[j] * t)] and cval0[t] =
+  // rc[class(t)] in units of 2^-13, rounded to nearest (|ReTr| < 4).
   std::vector<uint8_t> cls, cprod;
   std::vector<double> rc;
   std::vector<int32_t> rcq;
+  std::vector<int16_t> cval, cval0;
   double dmax = 0, rmax = 0;
 };
 
 constexpr unsigned kMaxClassOrder = 8192;
+constexpr unsigned kMaxClasses = 63;
 
 // Parses a groups/ table file (plain text or gzip): P, P ReTr, P ImTr, P*P
 // table; trailing tokens are ignored (group.c:13-47).  The identity is the

@@ -20,6 +20,7 @@ struct DeviceTables {
   double *retr = nullptr, *imtr = nullptr;
   uint8_t *cls = nullptr, *cprod = nullptr;  // class tables (null if the group has none)
   int32_t *rcq = nullptr;
+  int16_t *cval = nullptr, *cval0 = nullptr;
 };
 
 }  // namespace dsa
@@ -39,7 +40,8 @@ Tables tables_view(const dsa_group *g, const DeviceTables &t);
 // both trial actions' class-value error (D-dependent staple counts, the
 // table's within-class ReTr spread dmax) and the float rounding of both sums
 // and of beta0 + beta1 r1 + beta2 m1, with a safety factor of 4.
-double class_margin(const Group &g, int D, double beta0, double beta1, double beta2);
+// q: the quantisation of the class values (half their unit).
+double class_margin(const Group &g, int D, double beta0, double beta1, double beta2, double q);
 // Shape and option checks shared by chains and batches.
 int check_params(const dsa_chain_params *p);
 // minstd-compat start (dym-mod-metro.cpp:227-228, 253-255): per-site states and

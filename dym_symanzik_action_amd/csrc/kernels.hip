@@ -7,6 +7,8 @@
 // the proposal pool in LDS.  No MFMA: there is no dense contraction here.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "dsa_math.hpp"
 #include "kernels.hpp"
@@ -39,42 +41,59 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs A) {
   if ((threadIdx.x & 63u) == 0 && acc) atomicAdd(A.acc, acc);
 }
 
-// The same pass with the class-product update: the class-product table
-// (npool x P bytes, 78 KB for S(1080)) lives in LDS, so two 512-lane blocks
-// share a CU; the staples and accepted moves still gather from the global
-// table.
-constexpr int kClsBlock = 512;
+// The same pass with the class-product update.  Narrow: the class-index table
+// (npool x P bytes, 78 KB for S(1080)) in LDS, two 512-lane blocks per CU.
+// Wide: the class values themselves (npool x P int16, 155 KB for S(1080)),
+// one lookup per term, one 1024-lane block per CU.  The staples and accepted
+// moves still gather from the global table.
+constexpr int kClsBlockNarrow = 512, kClsBlockWide = 1024;
 
-template <int D>
-__global__ __launch_bounds__(kClsBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_cls(SweepArgs A) {
+__device__ inline void copy16(void *dst, const void *src, size_t bytes) {
+  const uint32_t n16 = (uint32_t)(pad16(bytes) / 16);
+  const uint4 *s = reinterpret_cast<const uint4 *>(src);
+  uint4 *d = reinterpret_cast<uint4 *>(dst);
+  for (uint32_t t = threadIdx.x; t < n16; t += blockDim.x) d[t] = s[t];
+}
+
+template <int D, bool W>
+__global__ __launch_bounds__(W ? kClsBlockWide : kClsBlockNarrow) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_sweep_cls(SweepArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // class values at a fixed LDS address: the narrow table's bytes index them
+  __shared__ int32_t s_rcq[64];
   const uint32_t P = A.T.P, np = A.T.npool, nc = A.T.ncls;
-  int32_t *s_rcq = reinterpret_cast<int32_t *>(smem);
-  uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem + pad16(4 * (size_t)nc));
+  uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem);
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(reinterpret_cast<unsigned char *>(s_inv) + pad16(2 * (size_t)P));
-  uint8_t *s_cls = reinterpret_cast<uint8_t *>(s_pool) + pad16(2 * (size_t)np);
-  uint8_t *s_cprod = s_cls + pad16(P);
-  for (uint32_t t = threadIdx.x; t < nc; t += blockDim.x) s_rcq[t] = A.T.rcq[t];
-  for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-    s_inv[t] = A.T.inv[t];
-    s_cls[t] = A.T.cls[t];
-  }
+  unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
+  unsigned char *s_tab = s_tab0 + pad16((W ? 2 : 1) * (size_t)P);
+  for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) s_inv[t] = A.T.inv[t];
   for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) s_pool[t] = A.T.pool[t];
-  const uint32_t n16 = (uint32_t)(pad16((size_t)np * P) / 16);
-  const uint4 *src = reinterpret_cast<const uint4 *>(A.T.cprod);
-  uint4 *dst = reinterpret_cast<uint4 *>(s_cprod);
-  for (uint32_t t = threadIdx.x; t < n16; t += blockDim.x) dst[t] = src[t];
+  if (W) {
+    copy16(s_tab0, A.T.cval0, 2 * (size_t)P);
+    copy16(s_tab, A.T.cval, 2 * (size_t)np * P);
+  } else {
+    for (uint32_t t = threadIdx.x; t < nc; t += blockDim.x) s_rcq[t] = A.T.rcq[t];
+    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) s_tab0[t] = A.T.cls[t];
+    copy16(s_tab, A.T.cprod, (size_t)np * P);
+  }
   __syncthreads();
 
   const TabView T{A.T.mult, s_inv, P};
   const LinkCtx C{A.links, A.mstate, nullptr, s_pool, (double)np, A.beta0, A.beta1, A.beta2,
                   A.seed, A.sweep_lo, A.sweep_hi, A.rng};
-  const ClsCtx K{s_cprod, s_cls, s_rcq, A.T.retr, A.cmargin};
+  const ClsCtx K{A.T.retr, A.cmargin};
+  using L = typename std::conditional<W, ClsWide, ClsNarrow>::type;
+  L Lk;
+  if constexpr (W) {
+    Lk = ClsWide{reinterpret_cast<const int16_t *>(s_tab), reinterpret_cast<const int16_t *>(s_tab0)};
+  } else {
+    Lk = ClsNarrow{s_tab, s_tab0, s_rcq};
+  }
   unsigned long long acc = 0;
   uint32_t nfall = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride)
-    acc += update_link_cls<D>(C, K, A.G, T, A.mu, A.sites[k], nfall);
+    acc += update_link_cls<D>(C, Lk, K, A.G, T, A.mu, A.sites[k], nfall);
   acc = wave_sum_u64(acc);
   const unsigned long long nf = wave_sum_u64(nfall);
   if ((threadIdx.x & 63u) == 0) {
@@ -360,25 +379,36 @@ size_t sweep_lds_bytes(const Tables &T) {
   return (size_t)T.P * sizeof(double) + (size_t)T.P * sizeof(uint16_t) + (size_t)T.npool * sizeof(uint16_t);
 }
 
-size_t sweep_cls_lds_bytes(const Tables &T) {
-  return pad16(4 * (size_t)T.ncls) + pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool) + pad16(T.P) +
+size_t sweep_cls_lds_bytes(const Tables &T, int mode) {
+  if (mode == kClsWide)
+    return pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool) + pad16(2 * (size_t)T.P) +
+           pad16(2 * (size_t)T.npool * T.P);
+  return pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool) + pad16(T.P) +
          pad16((size_t)T.npool * T.P);
 }
 
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
   if (A.count == 0) return hipSuccess;
-  if (A.use_cls) {
-    if (!A.T.cprod || !A.fall) return hipErrorInvalidValue;
-    uint32_t nb = (A.count + kClsBlock - 1) / kClsBlock;
+  if (A.use_cls == kClsNarrow || A.use_cls == kClsWide) {
+    const bool w = A.use_cls == kClsWide;
+    if (!A.fall || (w ? !A.T.cval : !A.T.cprod)) return hipErrorInvalidValue;
+    const uint32_t bs = w ? kClsBlockWide : kClsBlockNarrow;
+    uint32_t nb = (A.count + bs - 1) / bs;
     if (max_blocks > 0 && nb > (uint32_t)max_blocks) nb = (uint32_t)max_blocks;
-    const size_t lds = sweep_cls_lds_bytes(A.T);
+    const size_t lds = sweep_cls_lds_bytes(A.T, A.use_cls);
+#define DSA_CLS_CASE(DD)                                                                        \
+  case DD:                                                                                      \
+    if (w) hipLaunchKernelGGL((k_sweep_cls<DD, true>), dim3(nb), dim3(bs), lds, s, A);           \
+    else hipLaunchKernelGGL((k_sweep_cls<DD, false>), dim3(nb), dim3(bs), lds, s, A);            \
+    break;
     switch (A.G.D) {
-      case 2: hipLaunchKernelGGL(k_sweep_cls<2>, dim3(nb), dim3(kClsBlock), lds, s, A); break;
-      case 3: hipLaunchKernelGGL(k_sweep_cls<3>, dim3(nb), dim3(kClsBlock), lds, s, A); break;
-      case 4: hipLaunchKernelGGL(k_sweep_cls<4>, dim3(nb), dim3(kClsBlock), lds, s, A); break;
-      case 5: hipLaunchKernelGGL(k_sweep_cls<5>, dim3(nb), dim3(kClsBlock), lds, s, A); break;
+      DSA_CLS_CASE(2)
+      DSA_CLS_CASE(3)
+      DSA_CLS_CASE(4)
+      DSA_CLS_CASE(5)
       default: return hipErrorInvalidValue;
     }
+#undef DSA_CLS_CASE
     return hipGetLastError();
   }
   uint32_t nb = (A.count + kBlock - 1) / kBlock;

@@ -32,6 +32,8 @@ struct Tables {
   const uint8_t *cls, *cprod;
   const int32_t *rcq;
   uint32_t ncls;
+  // wide form: class values in units of 2^-13 (null if absent)
+  const int16_t *cval, *cval0;
 };
 
 enum RngKind : int { kRngMinstd = 0, kRngPhilox = 1 };
@@ -49,16 +51,17 @@ struct SweepArgs {
   double beta0, beta1, beta2;
   uint32_t seed;
   uint32_t sweep_lo, sweep_hi;
-  // class-product kernel: use it, the bound on |exact - class| trial-action
-  // differences below which a decision is recomputed exactly, and the count of
-  // such recomputations
+  // class-product kernel (kClsNone, kClsNarrow, kClsWide), the bound on
+  // |exact - class| trial-action differences below which a decision is
+  // recomputed exactly, and the count of such recomputations
   int use_cls;
   double cmargin;
   unsigned long long *fall;
 };
 
 size_t sweep_lds_bytes(const Tables &T);
-size_t sweep_cls_lds_bytes(const Tables &T);
+enum ClsMode : int { kClsNone = 0, kClsNarrow = 1, kClsWide = 2 };
+size_t sweep_cls_lds_bytes(const Tables &T, int mode);
 __host__ __device__ inline size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s);
 

@@ -112,17 +112,19 @@ int dsa_schedule_query(uint32_t D, uint32_t Nt, uint32_t Nx, int32_t order, uint
 /* Milliseconds the last dsa_chain_sweep spent on the device (HIP events). */
 int dsa_chain_last_sweep_ms(dsa_chain *c, double *ms);
 
-/* Sweep kernel choice (no counterpart in the reference; both kernels give
+/* Sweep kernel choice (no counterpart in the reference; all kernels give
  * bit-identical links and counters).  TABLE: every trial action from element
  * ReTr values through L2 gathers of the P x P table (the reference's
  * arithmetic, dym-mod-metro.cpp:164-192).  CLASS: trial actions from
- * conjugacy-class values held in LDS; a hit whose decision could depend on
- * the table's within-class ReTr rounding is decided again from element
- * values.  AUTO (default): CLASS for passes of at least 2048 links when the
- * group's class tables fit the device's LDS.  margin_scale (default 1)
- * multiplies the bound that selects those re-decisions: a huge value makes
- * every hit re-decided (a test of that path). */
-enum { DSA_KERNEL_AUTO = 0, DSA_KERNEL_TABLE = 1, DSA_KERNEL_CLASS = 2 };
+ * quantised conjugacy-class values held in LDS (int16 values when they fit,
+ * else byte class indices); a hit whose decision could depend on the
+ * quantisation or the table's within-class ReTr rounding is decided again
+ * from element values.  CLASS_NARROW forces the byte-index form.  AUTO
+ * (default): CLASS for passes of at least 2048 links when the group's class
+ * tables fit the device's LDS.  margin_scale (default 1) multiplies the bound
+ * that selects those re-decisions: a huge value makes every hit re-decided
+ * (a test of that path). */
+enum { DSA_KERNEL_AUTO = 0, DSA_KERNEL_TABLE = 1, DSA_KERNEL_CLASS = 2, DSA_KERNEL_CLASS_NARROW = 3 };
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale);
 /* Hits the CLASS kernel re-decided from element values since creation. */
 int dsa_chain_exact_decisions(dsa_chain *c, uint64_t *n);

@@ -7,8 +7,9 @@
   binary's GMES output (cli_c1.txt);
 * Philox streams in the race-free colour order: bit-exact against the oracle;
 * FP64 observable sums: <= 1e-12 * sum|terms| against the reference's order;
-* statistics of the default mode: within 2 sigma (Gamma-method errors) of the
-  reference binary's runs (cli_stats_*.txt);
+* statistics of the default mode: within 3.5 sigma (Gamma-method errors) of
+  the reference binary's runs (cli_stats_*.txt); the four stream x order
+  combinations sample one ensemble;
 * at the 32^3 x 64 benchmark size: size-independent properties.
 """
 import glob
@@ -114,18 +115,18 @@ def test_cli_replay_reproduces_reference_output():
     ("mys1080-v4", 4, 5, 4, 0.0, 2.0, -0.1, 14, "hot", 2),
 ])
 @pytest.mark.parametrize("rng", ["philox", "minstd"])
-@pytest.mark.parametrize("kernel", ["table", "class", "class-all-exact"])
+@pytest.mark.parametrize("kernel", ["table", "class", "class-narrow", "class-all-exact"])
 def test_colour_order_matches_oracle(dsa, dgroups, oracle, ogroups, case, rng, kernel):
-    """Both sweep kernels against the oracle; class-all-exact widens the
+    """Every sweep kernel against the oracle; class-all-exact widens the
     class kernel's bound so that every hit takes its exact re-decision path."""
     name, D, Nt, Nx, b0, b1, b2, seed, start, nsw = case
     say("case", name, D, Nt, Nx, rng, kernel)
     o = oracle.OChain(ogroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour")
     with dsa.Chain(dgroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour") as c:
-        if kernel == "table":
-            c.set_kernel("table")
+        if kernel == "class-all-exact":
+            c.set_kernel("class", 1e300)
         else:
-            c.set_kernel("class", 1e300 if kernel == "class-all-exact" else 1.0)
+            c.set_kernel(kernel)
         assert np.array_equal(c.a, o.a), "start differs"
         for s in range(nsw):
             c.update(1)
@@ -150,8 +151,8 @@ def test_config2_shape_one_sweep_matches_oracle(dsa, dgroups, oracle, ogroups):
         o.sweep(1)
         assert np.array_equal(c.a, o.a)
         assert c.counters() == o.counters()
-        # re-decisions are rare: the bound is ~1e-5 in action units
-        assert c.exact_decisions() < 1e-4 * c.counters()[0]
+        # re-decisions are rare: the bound is ~3e-3 in action units
+        assert c.exact_decisions() < 1e-2 * c.counters()[0]
         # The reference's own one-thread order over 786,432 plaquettes carries
         # ~1e-12 relative rounding error itself, so the tree is held to the
         # near-exact per-element form sum_e count[e]*ReTr[e] of the oracle.
@@ -194,15 +195,19 @@ def _stats_compare(ref_lines, got_lines, nx, nt, keys):
 
 
 @pytest.mark.parametrize("fixture,args,nx,nt", [
-    # the reference ran seeds 9 and 19 at one thread; the GPU runs its default
-    # mode (Philox, colour order) with other seeds
-    ("cli_stats_4x4.txt", ["4", "4", "4", "0", "1.2", "-0.1", "93", "2", "1000", "50"], 4, 4),
-    ("cli_stats_6x6.txt", ["4", "6", "6", "0", "1.6", "-0.25", "94", "1", "600", "50"], 6, 6),
+    # the reference ran seeds 9 and 19 at one thread (1000 and 600
+    # measurements); the GPU runs its default mode (Philox, colour order) with
+    # other seeds and ten times the measurements
+    ("cli_stats_4x4.txt", ["4", "4", "4", "0", "1.2", "-0.1", "93", "2", "10000", "50"], 4, 4),
+    ("cli_stats_6x6.txt", ["4", "6", "6", "0", "1.6", "-0.25", "94", "1", "6000", "50"], 6, 6),
 ])
 def test_statistics_agree_with_reference(fixture, args, nx, nt):
     """Long single chains in a mixing regime (acceptance 0.2-0.3): means of the
-    plaquette, rectangle, |P| and Wilson loops agree within 2 sigma, sigma from
-    both chains' Gamma-method (autocorrelation-corrected) errors."""
+    plaquette, rectangle, |P| and Wilson loops agree within 3.5 sigma, sigma
+    from both chains' Gamma-method (autocorrelation-corrected) errors.  Eight
+    correlated observables are tested at once, so a 2-sigma cut would fail a
+    correct sampler about one run in three; a wrong action or staple shifts
+    these means by tens of sigma."""
     say("case stats", fixture)
     out = subprocess.run([CLI, group_file("mys1080-v4")] + args, capture_output=True, text=True, check=True,
                          timeout=900).stdout
@@ -211,7 +216,46 @@ def test_statistics_agree_with_reference(fixture, args, nx, nt):
     rep = _stats_compare(ref, out.splitlines(), nx, nt, keys)
     for k, m1, e1, m2, e2, z in rep:
         say(f"  {k}: ref {m1:.6f}+-{e1:.6f}  gpu {m2:.6f}+-{e2:.6f}  z={z:.2f}")
-    assert all(z < 2.0 for *_, z in rep), rep
+    assert all(z < 3.5 for *_, z in rep), rep
+
+
+def test_visiting_orders_and_streams_sample_the_same_ensemble(dsa, dgroups):
+    """The four combinations of stream (Philox, minstd-compat) and order (race-
+    free colours, the reference's one-thread order) are different Markov chains
+    with one stationary distribution: 4 x 32 independent 4^4 chains (beta =
+    (0, 1.2, -0.1), acceptance ~0.29), 100 + 2 x 2000 sweeps each, agree on
+    the plaquette and rectangle means to ~4e-5 (3.5 sigma of the difference).
+    minstd-compat seeds site i's engine with iseed + i (dym-mod-metro.cpp:
+    227-228), so chains with nearby seeds run the same engines one site apart
+    and are not independent: their seeds are spaced 1000003 apart."""
+    from stats_util import gamma_method
+
+    say("case orders x streams ensemble")
+    g = dgroups("mys1080-v4")
+    modes = [("philox", "colour"), ("philox", "reference"), ("minstd", "colour"), ("minstd", "reference")]
+    seeds = [lambda j: 30000 + j, lambda j: 31000 + j, lambda j: 1 + 1000003 * j, lambda j: 500001 + 1000003 * j]
+    n = 32
+    with dsa.Batch() as b:
+        for k, (rng, order) in enumerate(modes):
+            for j in range(n):
+                b.add(g, 4, 4, 4, 0.0, 1.2, -0.1, seeds[k](j), rng=rng, order=order)
+        b.finalize()
+        b.sweep(100)
+        out = b.run_cycles(2000, 2)
+    res = []
+    for k, mode in enumerate(modes):
+        sel = slice(k * n, (k + 1) * n)
+        row = []
+        for q in (2, 3):
+            errs = [gamma_method(out[:, c, q])[1] for c in range(k * n, (k + 1) * n)]
+            row.append((out[:, sel, q].mean(), np.sqrt(np.sum(np.square(errs))) / n))
+        res.append(row)
+        say(f"  {mode}: plaq {row[0][0]:.6f}+-{row[0][1]:.6f}  rect {row[1][0]:.6f}+-{row[1][1]:.6f}")
+    for q in range(2):
+        for a in range(4):
+            for c in range(a + 1, 4):
+                (m1, e1), (m2, e2) = res[a][q], res[c][q]
+                assert abs(m1 - m2) < 3.5 * np.hypot(e1, e2), (modes[a], modes[c], q, m1, m2, e1, e2)
 
 
 def test_frozen_config1_replicas_agree_with_reference(dsa, dgroups):
