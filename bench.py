@@ -136,6 +136,62 @@ def batched_vs_serial(dsa, sweeps):
             "batched_s": tb, "serial_s": ts, "speedup": ts / tb}
 
 
+def serving_requests(dsa, nreq, seed=2024):
+    """LLR window requests as a Poisson stream (50 requests/s).  Each asks for
+    a_n of one plaquette-energy window of width 1 % of the S1 range (NPLAQ *
+    4.5 for S(1080), NPLAQ = 6V), centred uniformly on S1/NPLAQ in [-1.9, -0.6]
+    (the greedy drive-in from a hot start stalls in metastable states below
+    about -2.1 on 8^4); lattice 8^4, 12^4 or 16^4; beta2 in {0, -0.1, -0.25};
+    hot start; drive-in cap 4000 sweeps (llr2d.cpp's); Robbins-Monro with K =
+    20 sweeps per iteration until |delta a| < 1e-4 or 50 iterations
+    (llr.cpp:223-250)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.concatenate([[0.0], np.cumsum(rng.exponential(1 / 50.0, nreq - 1))])
+    params, shapes = [], []
+    for r in range(nreq):
+        L = int(rng.choice([8, 12, 16]))
+        b2 = float(rng.choice([0.0, -0.1, -0.25]))
+        e = float(rng.uniform(-1.9, -0.6))
+        nplaq = 6 * L ** 4
+        a0 = 1.2 + (-0.7 - e) * (1.8 / 1.6)  # rough beta at which <S1>/NPLAQ = e
+        params.append(dsa.dsa.llr_params(4, L, L, 1, b2, e * nplaq, 0.005 * 4.5 * nplaq, a0, 1000 + r, K=20, NRM=50,
+                                         tol=1e-4, cap=4000))
+        shapes.append(L)
+    return arrivals, params, shapes
+
+
+def serving(dsa, nreq, slots):
+    """p50/p99 latency (arrival to converged a_n) and windows/s of the LLR
+    serving loop, and windows/s serving the same requests one at a time."""
+    import numpy as np
+
+    g = dsa.load_group(dsa.group_path("mys1080-v4"))
+    arrivals, params, shapes = serving_requests(dsa, nreq)
+    with dsa.dsa.LlrEngine(4, slots) as eng:
+        t0 = time.perf_counter()
+        admit, done, st = eng.serve([g] * nreq, params, arrivals, chunk=5)
+        makespan = time.perf_counter() - t0
+    log(f"bench: served {nreq} windows in {makespan:.1f} s")
+    lat = done - arrivals
+    t_serial = 0.0
+    for r, p in enumerate(params):
+        with dsa.dsa.LlrEngine(4, 1) as eng:
+            t0 = time.perf_counter()
+            eng.serve([g], [p], [0.0], chunk=5)
+            t_serial += time.perf_counter() - t0
+        log(f"bench: window {r + 1}/{nreq} alone, {t_serial:.1f} s so far")
+    return {"workload": f"{nreq} LLR window requests, Poisson 50/s, lattices 8^4/12^4/16^4 "
+                        f"({shapes.count(8)}/{shapes.count(12)}/{shapes.count(16)}), beta2 in {{0,-0.1,-0.25}}, "
+                        "window 1% of the S1 range, K=20, NRM=50, tol 1e-4",
+            "slots": slots, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "windows_per_s": nreq / makespan,
+            "serial_windows_per_s": nreq / t_serial, "speedup": t_serial / makespan,
+            "reached": int(sum(s["reached"] for s in st)),
+            "mean_rm_iterations": float(np.mean([s["iterations"] for s in st]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +200,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scan-sweeps", type=int, default=10,
                     help="sweeps of the 256-chain scan timed batched and serially (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=12,
+                    help="LLR window requests for the serving measurement (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--kernel", choices=["auto", "table", "class", "class-narrow"], default="auto",
                     help="sweep kernel (identical results): element values via L2 gathers, or class values in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -205,6 +264,11 @@ def main():
         log(f"bench: 256-chain scan, {args.scan_sweeps} sweeps batched and one chain at a time")
         scan = batched_vs_serial(dsa, args.scan_sweeps)
 
+    serve = None
+    if args.serve_requests:
+        log(f"bench: LLR serving, {args.serve_requests} requests, {args.serve_slots} slots, then one at a time")
+        serve = serving(dsa, args.serve_requests, args.serve_slots)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -242,6 +306,7 @@ def main():
                                 "per_unit": GATHERS_PER_LINK_UPDATE}},
         "cpu_baseline": cpu,
         "batched_vs_serial": scan,
+        "llr_serving": serve,
     }
     if copy_ms is not None:
         out["copy_probe"] = {"bytes": args.copy_probe_mb << 20, "ms": copy_ms}

@@ -28,6 +28,14 @@ __device__ inline long long wave_sum_i64(long long v) {
   return v;
 }
 
+
+// lane k's value (k uniform over the wave)
+__device__ inline long long readlane64(long long v, uint32_t k) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((unsigned long long)v >> 32), (int)k);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ inline bool llr_active(uint32_t phase) { return phase == kLlrSeed || phase == kLlrRM; }
 
 // drive-in: is the windowed energy outside the window at the start of a pass?
@@ -140,8 +148,25 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
   }
 }
 
-// One wave per chain.  Chunks of 64 proposals are accepted at once when every
-// prefix of the chunk obeys the rule; otherwise lane 0 walks the chunk.
+// One wave per chain, in chunks of 64 proposals held in registers (eight
+// chunks are loaded at a time so the loads overlap).  With inclusive prefix
+// sums of the energy changes, lane k knows the running energy if every
+// proposal from the chunk's first undecided one up to k is accepted; the
+// first lane whose rule fails is refused (its link restored), all lanes
+// before it are accepted, and the scan resumes after it.  This is the
+// sequential rule exactly: each decision sees the energy after all earlier
+// accepted proposals.  A chunk costs one step plus one per refusal.
+constexpr int kDecideSlab = 8;
+
+__device__ inline long long scan_i64(long long v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
 __global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
   const uint32_t c = blockIdx.x;
   const int lane = (int)threadIdx.x;
@@ -155,49 +180,44 @@ __global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrSt
   long long E1 = S.S1, E2 = S.S2;
   unsigned long long acc = 0;
   const Geom &G = LC.base.G;
-  for (uint32_t base = 0; base < pr.count; base += 64) {
-    const uint32_t j = base + (uint32_t)lane;
-    const bool valid = j < pr.count;
-    const uint32_t idx = pr.offset + j;
-    const long long d1 = valid ? LC.dE1[idx] : 0;
-    const long long d2 = valid ? LC.dE2[idx] : 0;
-    const long long dw = ax1 ? d1 : d2;
-    const unsigned long long na = valid ? LC.nacc[idx] : 0;
-    long long p = dw;
+  for (uint32_t slab = 0; slab < pr.count; slab += 64 * kDecideSlab) {
+    long long r1[kDecideSlab], r2[kDecideSlab];
+    uint32_t rn[kDecideSlab];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const long long t = __shfl_up(p, o, 64);
-      if (lane >= o) p += t;
+    for (int q = 0; q < kDecideSlab; ++q) {
+      const uint32_t j = slab + 64u * q + (uint32_t)lane;
+      const bool valid = j < pr.count;
+      r1[q] = valid ? LC.dE1[pr.offset + j] : 0;
+      r2[q] = valid ? LC.dE2[pr.offset + j] : 0;
+      rn[q] = valid ? LC.nacc[pr.offset + j] : 0;
     }
-    const long long Ew = ax1 ? E1 : E2;
-    const bool ok = !valid || llr_rule(drive, lo, hi, Ew + p - dw, Ew + p);
-    if (__all(ok)) {
-      E1 += wave_sum_i64(d1);
-      E2 += wave_sum_i64(d2);
-      E1 = __shfl(E1, 0, 64);
-      E2 = __shfl(E2, 0, 64);
-      acc += __shfl(wave_sum_u64(na), 0, 64);
-    } else {
-      if (lane == 0) {
-        const uint32_t n = pr.count - base < 64 ? pr.count - base : 64;
-        for (uint32_t q = 0; q < n; ++q) {
-          const uint32_t k = pr.offset + base + q;
-          const long long e1 = LC.dE1[k], e2 = LC.dE2[k];
-          const long long ew = ax1 ? e1 : e2;
-          const long long Ecur = ax1 ? E1 : E2;
-          if (llr_rule(drive, lo, hi, Ecur, Ecur + ew)) {
-            E1 += e1;
-            E2 += e2;
-            acc += LC.nacc[k];
-          } else {
-            const uint32_t site = LC.base.sites[k];
-            LC.base.links[(size_t)pr.mu * G.V + lpos(G, site, site_parity(G, site))] = LC.old[k];
-          }
+#pragma unroll
+    for (int q = 0; q < kDecideSlab; ++q) {
+      const uint32_t base = slab + 64u * q;
+      if (base >= pr.count) break;
+      const uint32_t n = pr.count - base < 64 ? pr.count - base : 64;
+      const long long dw = ax1 ? r1[q] : r2[q];
+      const long long P1 = scan_i64(r1[q], lane), P2 = scan_i64(r2[q], lane);
+      const long long PN = scan_i64((long long)rn[q], lane);
+      const long long Pw = ax1 ? P1 : P2;
+      for (uint32_t s0 = 0; s0 < n;) {  // s0 is uniform over the wave
+        const long long Ew = ax1 ? E1 : E2;
+        const long long Enew = Ew + Pw - (s0 ? readlane64(Pw, s0 - 1) : 0);
+        const bool ok = (uint32_t)lane < s0 || (uint32_t)lane >= n || llr_rule(drive, lo, hi, Enew - dw, Enew);
+        const unsigned long long bad = __ballot(!ok);
+        const uint32_t f = bad ? (uint32_t)__builtin_ctzll(bad) : n;
+        if (f > s0) {  // accept [s0, f)
+          E1 += readlane64(P1, f - 1) - (s0 ? readlane64(P1, s0 - 1) : 0);
+          E2 += readlane64(P2, f - 1) - (s0 ? readlane64(P2, s0 - 1) : 0);
+          acc += (unsigned long long)(readlane64(PN, f - 1) - (s0 ? readlane64(PN, s0 - 1) : 0));
         }
+        if (f < n && (uint32_t)lane == f) {
+          const uint32_t idx = pr.offset + base + f;
+          const uint32_t site = LC.base.sites[idx];
+          LC.base.links[(size_t)pr.mu * G.V + lpos(G, site, site_parity(G, site))] = LC.old[idx];
+        }
+        s0 = f + 1;
       }
-      E1 = __shfl(E1, 0, 64);
-      E2 = __shfl(E2, 0, 64);
-      acc = __shfl(acc, 0, 64);
     }
   }
   if (lane == 0) {
