@@ -553,6 +553,83 @@ int dsa_chain_set_links(dsa_chain *c, const int32_t *a) {
   return upload_links(c, a);
 }
 
+int dsa_group_matrices(const dsa_group *g, double *out) {
+  if (!g || !out) return fail(DSA_E_ARG, "dsa_group_matrices: NULL argument");
+  if (g->g.su3.empty()) return fail(DSA_E_STATE, "group file " + g->g.path + " has no appended SU(3) matrices");
+  memcpy(out, g->g.su3.data(), g->g.su3.size() * sizeof(double));
+  return 0;
+}
+
+int dsa_cfg_save(const char *path, const int32_t *a, uint64_t n) {
+  if (!path || (!a && n)) return fail(DSA_E_ARG, "dsa_cfg_save: NULL argument");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(DSA_E_IO, std::string("dsa_cfg_save: cannot open ") + path);
+  const size_t w = fwrite(a, sizeof(int32_t), (size_t)n, f);
+  const int cl = fclose(f);
+  if (w != (size_t)n || cl) return fail(DSA_E_IO, std::string("dsa_cfg_save: short write to ") + path);
+  return 0;
+}
+
+int dsa_cfg_load(const char *path, int32_t *a, uint64_t n) {
+  if (!path || (!a && n)) return fail(DSA_E_ARG, "dsa_cfg_load: NULL argument");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(DSA_E_IO, std::string("dsa_cfg_load: cannot open ") + path);
+  const size_t r = fread(a, sizeof(int32_t), (size_t)n, f);
+  const bool more = fgetc(f) != EOF;
+  fclose(f);
+  if (r != (size_t)n || more)
+    return fail(DSA_E_RANGE, std::string("dsa_cfg_load: ") + path + " does not hold exactly " + std::to_string(n) +
+                                 " int32 links");
+  return 0;
+}
+
+int dsa_chain_save_cfg(dsa_chain *c, const char *path) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_save_cfg: NULL chain");
+  std::vector<int32_t> a((size_t)c->G.V * c->G.D);
+  if (int rc = dsa_chain_get_links(c, a.data())) return rc;
+  return dsa_cfg_save(path, a.data(), a.size());
+}
+
+int dsa_chain_load_cfg(dsa_chain *c, const char *path) {
+  if (!c) return fail(DSA_E_ARG, "dsa_chain_load_cfg: NULL chain");
+  std::vector<int32_t> a((size_t)c->G.V * c->G.D);
+  if (int rc = dsa_cfg_load(path, a.data(), a.size())) return rc;
+  return dsa_chain_set_links(c, a.data());
+}
+
+int dsa_chain_save_kentucky(dsa_chain *c, const char *prefix, int32_t num, char *name, uint64_t cap) {
+  if (!c || !prefix) return fail(DSA_E_ARG, "dsa_chain_save_kentucky: NULL argument");
+  if (c->G.D != 4) return fail(DSA_E_STATE, "dsa_chain_save_kentucky: the Kentucky layout is four-dimensional");
+  const std::vector<double> &m = c->grp->g.su3;
+  if (m.empty()) return fail(DSA_E_STATE, "group file " + c->grp->g.path + " has no appended SU(3) matrices");
+  if (set_device(c)) return DSA_E_DEVICE;
+  char fname[1024];
+  snprintf(fname, sizeof fname, "%slat-beta1%.3f-beta2%.3f-nt%02d-nx%02d-num%04d", prefix, c->p.beta1, c->p.beta2,
+           (int)c->p.Nt, (int)c->p.Nx, (int)num);
+  const size_t nd = (size_t)c->G.V * 4 * 18;
+  double *d_out = nullptr, *d_m = nullptr;
+  std::vector<double> host(nd);
+  hipError_t e = hipMalloc(&d_out, nd * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&d_m, m.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(d_m, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = dsa::launch_kentucky(d_out, c->d_links, d_m, c->G, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(host.data(), d_out, nd * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_out) (void)hipFree(d_out);
+  if (d_m) (void)hipFree(d_m);
+  if (e != hipSuccess) return fail(DSA_E_DEVICE, std::string("dsa_chain_save_kentucky: ") + hipGetErrorString(e));
+  FILE *f = fopen(fname, "wb");
+  if (!f) return fail(DSA_E_IO, std::string("dsa_chain_save_kentucky: cannot open ") + fname);
+  const size_t w = fwrite(host.data(), sizeof(double), nd, f);
+  const int cl = fclose(f);
+  if (w != nd || cl) return fail(DSA_E_IO, std::string("dsa_chain_save_kentucky: short write to ") + fname);
+  if (name && cap) {
+    strncpy(name, fname, (size_t)cap - 1);
+    name[cap - 1] = 0;
+  }
+  return 0;
+}
+
 int dsa_chain_get_rng_state(dsa_chain *c, uint32_t *state) {
   if (!c || !state) return fail(DSA_E_ARG, "dsa_chain_get_rng_state: NULL argument");
   if (!c->d_mstate) return fail(DSA_E_STATE, "chain does not use the minstd streams");

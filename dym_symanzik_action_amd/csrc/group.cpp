@@ -66,6 +66,18 @@ std::string load_group(const char *path, Group &g) {
     g.mult[n] = (uint16_t)v;
     p = e;
   }
+  // optional: the defining-representation matrices appended after the table,
+  // 18 reals per element (dym-mod-metro-savecfg.cpp:249-262)
+  {
+    std::vector<double> m((size_t)P * 18);
+    size_t n = 0;
+    for (; n < m.size(); ++n) {
+      m[n] = strtod(p, &e);
+      if (e == p) break;
+      p = e;
+    }
+    if (n == m.size()) g.su3.swap(m);
+  }
   text.clear();
   text.shrink_to_fit();
   // identity: first n with identity row and column

@@ -18,6 +18,9 @@ struct Group {
   std::vector<uint16_t> mult;      // P*P, mult[n*P+m] = n*m
   std::vector<uint16_t> inv;       // P
   std::vector<uint16_t> pool;      // proposal elements (nearest the identity)
+  // defining-representation (SU(3)) matrix of each element when the file
+  // appends them: su3[18*e + part + 2*(c + 3*r)] (empty otherwise)
+  std::vector<double> su3;
   double min_retr = 0, nn_retr = 0;
   // Conjugacy classes, for the class-product sweep kernel (empty when the
   // group has more than kMaxClasses classes or P > kMaxClassOrder).  rc[c]

@@ -171,6 +171,27 @@ __global__ void k_unpack(int32_t *aos, const uint16_t *links, Geom G) {
   }
 }
 
+__global__ void k_kentucky(unsigned long long *out, const uint16_t *links, const double *mats, Geom G) {
+  const uint32_t Nt = G.ext[0], Nx = G.ext[1];
+  for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < G.V; o += gridDim.x * blockDim.x) {
+    // o = x1 + Nx*(x2 + Nx*(x3 + Nx*x0)); site i = x0 + Nt*(x1 + Nx*(x2 + Nx*x3))
+    const uint32_t x1 = o % Nx, x2 = (o / Nx) % Nx, x3 = (o / (Nx * Nx)) % Nx, x0 = o / (Nx * Nx * Nx);
+    const uint32_t i = x0 + Nt * (x1 + Nx * (x2 + Nx * x3));
+    const uint32_t par = (x0 + x1 + x2 + x3) & 1u;
+    for (int d = 0; d < 4; ++d) {
+      const uint32_t e = links[(size_t)d * G.V + lpos(G, i, par)];
+      const uint32_t dd = (uint32_t)((d + 3) % 4);
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+          for (int part = 0; part < 2; ++part) {
+            const double v = mats[18u * e + part + 2 * (c + 3 * r)];
+            const size_t k = o + (size_t)G.V * (part + 2 * (r + 3 * (c + 3 * dd)));
+            out[k] = __builtin_bswap64(__double_as_longlong(v));
+          }
+    }
+  }
+}
+
 template <int D>
 __global__ void k_staples(int32_t *st_out, int32_t *re_out, const uint16_t *links, Tables Tb, Geom G) {
   constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1);
@@ -421,6 +442,13 @@ hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
     case 5: hipLaunchKernelGGL(k_sweep<5>, dim3(nb), dim3(kBlock), lds, s, A); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_kentucky(double *out, const uint16_t *links, const double *mats, const Geom &G, hipStream_t s) {
+  if (G.D != 4) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_kentucky, dim3(grid_for(G.V)), dim3(kBlock), 0, s, reinterpret_cast<unsigned long long *>(out),
+                     links, mats, G);
   return hipGetLastError();
 }
 

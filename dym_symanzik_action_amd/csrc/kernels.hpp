@@ -160,6 +160,11 @@ hipError_t launch_fill(uint16_t *links, const Geom &G, uint16_t v, hipStream_t s
 hipError_t launch_pack(uint16_t *links, const int32_t *aos, const Geom &G, hipStream_t s);
 hipError_t launch_unpack(int32_t *aos, const uint16_t *links, const Geom &G, hipStream_t s);
 
+// SU(3) matrices of every link in the Kentucky layout (D = 4): out[o + V*(part
+// + 2*(r + 3*(c + 3*((d+3)%4))))] for output site o = x1 + Nx*(x2 + Nx*(x3 +
+// Nx*x0)), each double byte-swapped to big-endian; mats[18*e + part + 2*(c+3r)].
+hipError_t launch_kentucky(double *out, const uint16_t *links, const double *mats, const Geom &G, hipStream_t s);
+
 // Staples of every link (parity checks): st[(i*D+mu)*2(D-1)+j], re[...*6(D-1)+k]
 hipError_t launch_staples(int32_t *st, int32_t *re, const uint16_t *links, const Tables &T, const Geom &G,
                           hipStream_t s);

@@ -97,6 +97,12 @@ _SIGS = {
     "dsa_batch_last_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "dsa_chain_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "dsa_chain_set_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "dsa_group_matrices": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dsa_cfg_save": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64]),
+    "dsa_cfg_load": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64]),
+    "dsa_chain_save_cfg": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "dsa_chain_load_cfg": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "dsa_chain_save_kentucky": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_uint64]),
     "dsa_chain_exact_decisions": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "dsa_chain_kernel_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "dsa_probe_gather": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
@@ -161,6 +167,15 @@ class Group:
         pool = np.empty(self.npool, np.int32)
         _check(lib().dsa_group_tables(self._h, _ptr(mult), _ptr(inv), _ptr(retr), _ptr(imtr), _ptr(pool)))
         return mult, inv, retr, imtr, pool
+
+    def matrices(self) -> np.ndarray:
+        """The defining-representation matrices appended to the group file, as
+        complex [P, 3, 3] (dym-mod-metro-savecfg.cpp:249-262); raises if the
+        file has none."""
+        raw = np.empty(self.P * 18, np.float64)
+        _check(lib().dsa_group_matrices(self._h, _ptr(raw)))
+        m = raw.reshape(self.P, 3, 3, 2)
+        return m[..., 0] + 1j * m[..., 1]
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -227,6 +242,21 @@ class Chain:
     @sweeps_done.setter
     def sweeps_done(self, v: int) -> None:
         _check(lib().dsa_chain_set_sweep(self._h, v))
+
+    def save_cfg(self, path: str) -> None:
+        """Raw int32 configuration file (llr2d.cpp:262-265)."""
+        _check(lib().dsa_chain_save_cfg(self._h, os.fsencode(path)))
+
+    def load_cfg(self, path: str) -> None:
+        """Loads a raw int32 configuration file (llr2d.cpp:266-271)."""
+        _check(lib().dsa_chain_load_cfg(self._h, os.fsencode(path)))
+
+    def save_kentucky(self, prefix: str, num: int) -> str:
+        """SU(3) configuration in savekyconfig's layout and byte order
+        (dym-mod-metro-savecfg.cpp:55-86); returns the file name."""
+        name = C.create_string_buffer(1024)
+        _check(lib().dsa_chain_save_kentucky(self._h, os.fsencode(prefix), num, name, 1024))
+        return name.value.decode()
 
     def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
         """Sweep kernel: "table" (element ReTr via L2 gathers), "class" (class

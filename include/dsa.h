@@ -66,6 +66,10 @@ int dsa_group_info_get(const dsa_group *g, dsa_group_info *info);
 /* Copies the tables behind the globals ReTr, ImTr, mult, inv (group.h:10-14);
  * any pointer may be NULL.  mult is P*P int32 row-major, pool is npool int32. */
 int dsa_group_tables(const dsa_group *g, int32_t *mult, int32_t *inv, double *retr, double *imtr, int32_t *pool);
+/* The defining-representation (SU(3)) matrices appended to a group file after
+ * its table, as dym-mod-metro-savecfg.cpp:249-262 reads them: out[18*e + part
+ * + 2*(c + 3*r)], P*18 doubles.  DSA_E_STATE if the file has none. */
+int dsa_group_matrices(const dsa_group *g, double *out);
 void dsa_group_free(dsa_group *g);
 
 /* Replaces main's set-up (dym-mod-metro.cpp:222-268): volume, per-site
@@ -111,6 +115,24 @@ int dsa_schedule_query(uint32_t D, uint32_t Nt, uint32_t Nx, int32_t order, uint
                        uint32_t *pass_mu, uint32_t *pass_count, uint32_t *sites);
 /* Milliseconds the last dsa_chain_sweep spent on the device (HIP events). */
 int dsa_chain_last_sweep_ms(dsa_chain *c, double *ms);
+
+/* ---- configuration files ----------------------------------------------------
+ * Raw configuration: the int32 link array a[i*D+mu] in host byte order, V*D
+ * entries, no header -- llr2d's save_cfg/load_cfg (llr2d.cpp:262-271).  Load
+ * fails (DSA_E_RANGE) unless the file holds exactly n entries, and rejects
+ * entries outside [0, P) on a chain. */
+int dsa_cfg_save(const char *path, const int32_t *a, uint64_t n);
+int dsa_cfg_load(const char *path, int32_t *a, uint64_t n);
+int dsa_chain_save_cfg(dsa_chain *c, const char *path);
+int dsa_chain_load_cfg(dsa_chain *c, const char *path);
+/* Writes the chain's configuration as SU(3) matrices in the Kentucky byte
+ * order that dym-mod-metro-savecfg.cpp:55-86 (savekyconfig) writes: V*4*18
+ * big-endian doubles, link (x, d) element (r, c, part) at index x1 + Nx*(x2 +
+ * Nx*(x3 + Nx*(x0 + Nt*(part + 2*(r + 3*(c + 3*((d+3)%4))))))), to
+ * "<prefix>lat-beta1%.3f-beta2%.3f-nt%02d-nx%02d-num%04d".  D = 4 and a group
+ * file with appended matrices only.  The file name is copied to name (cap
+ * bytes) when name is not NULL.  The matrices are expanded on the device. */
+int dsa_chain_save_kentucky(dsa_chain *c, const char *prefix, int32_t num, char *name, uint64_t cap);
 
 /* Sweep kernel choice (no counterpart in the reference; all kernels give
  * bit-identical links and counters).  TABLE: every trial action from element

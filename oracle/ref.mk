@@ -7,6 +7,7 @@
 # Products:
 #   _ref/dym-mod-metro   the reference driver, flags as in its Makefile:1-3
 #   _ref/llr, _ref/llr2d the LLR drivers (same flags)
+#   _ref/dym-mod-metro-savecfg  the driver that also writes SU(3) configurations
 #   _ref/ref_harness     ref_harness.cpp (ours) linked against the reference's
 #                        own get_staples / get_rect_loops / update (main renamed)
 #   _ref/verify_group    the reference's table checker
@@ -21,7 +22,8 @@ LDLIBS = -lm
 
 SRC_C = $(REF)/lattice.c $(REF)/group.c
 
-all: $(OUT)/dym-mod-metro $(OUT)/llr $(OUT)/llr2d $(OUT)/ref_harness $(OUT)/verify_group
+all: $(OUT)/dym-mod-metro $(OUT)/llr $(OUT)/llr2d $(OUT)/ref_harness $(OUT)/verify_group \
+     $(OUT)/dym-mod-metro-savecfg
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -40,6 +42,8 @@ $(OUT)/dym-mod-metro: $(REF)/dym-mod-metro.cpp $(REF_OBJS)
 $(OUT)/llr: $(REF)/llr.cpp $(REF_OBJS)
 	g++ $(REF_CXXFLAGS) -w -I$(REF) $< $(REF_OBJS) $(LDLIBS) -o $@
 $(OUT)/llr2d: $(REF)/llr2d.cpp $(REF_OBJS)
+	g++ $(REF_CXXFLAGS) -w -I$(REF) $< $(REF_OBJS) $(LDLIBS) -o $@
+$(OUT)/dym-mod-metro-savecfg: $(REF)/dym-mod-metro-savecfg.cpp $(REF_OBJS)
 	g++ $(REF_CXXFLAGS) -w -I$(REF) $< $(REF_OBJS) $(LDLIBS) -o $@
 $(OUT)/verify_group: $(REF)/verify_group.c $(OUT)/group.o
 	gcc $(REF_CFLAGS) -w -I$(REF) $< $(OUT)/group.o $(LDLIBS) -o $@

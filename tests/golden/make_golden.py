@@ -110,41 +110,88 @@ def llr_fixture(name, args):
         f.write(out)
 
 
+def kentucky_fixture(name, group, head, seed, K, N):
+    """SU(3) configurations written by the reference's savecfg driver
+    (dym-mod-metro-savecfg.cpp:55-86) after every K sweeps, N of them."""
+    with tempfile.TemporaryDirectory() as td:
+        run([os.path.join(REFBIN, "dym-mod-metro-savecfg"), os.path.join(REF, "groups", group)] + head +
+            [str(seed), td + "/", str(K), str(N), "0"], cwd=td)
+        files = sorted(f for f in os.listdir(td) if f.startswith("lat-"))
+        data = {f"file{n}": np.frombuffer(open(os.path.join(td, f), "rb").read(), np.uint8) for n, f in enumerate(files)}
+    np.savez_compressed(os.path.join(HERE, f"kentucky_{name}.npz"), group=group, args=np.array(head),
+                        seed=seed, K=K, N=N, names=np.array(files), **data)
+
+
+def llr2d_fixture(name, group, D, Nt, Nx, seed_args, solve_args):
+    """llr2d's seed pass (drive-in of every cell, CELL: manifest, raw int32
+    .cfg per cell: llr2d.cpp:262-271, 416-474) and a solve pass over the
+    manifest (ANE2: lines, llr2d.cpp:476-512)."""
+    with tempfile.TemporaryDirectory() as td:
+        cfgdir = os.path.join(td, "cfg")
+        os.makedirs(cfgdir)
+        out = run([os.path.join(REFBIN, "llr2d"), "seed", os.path.join(REF, "groups", group), str(D), str(Nt), str(Nx)]
+                  + seed_args + [cfgdir], cwd=td)
+        cells = [l for l in out.splitlines() if l.startswith("CELL:")]
+        man = os.path.join(td, "manifest.txt")
+        open(man, "w").write("\n".join(cells) + "\n")
+        sol = run([os.path.join(REFBIN, "llr2d"), "solve", os.path.join(REF, "groups", group), str(D), str(Nt), str(Nx),
+                   man, "0", str(len(cells))] + solve_args, cwd=td)
+        cfgs = {}
+        for l in cells:
+            path = l.split()[-1]
+            cfgs[os.path.basename(path)[:-4]] = np.fromfile(path, np.int32)
+    rel = [l.replace(cfgdir + "/", "cfg/") for l in cells]
+    np.savez_compressed(os.path.join(HERE, f"llr2d_{name}.npz"), group=group, D=D, Nt=Nt, Nx=Nx,
+                        seed_args=np.array(seed_args), solve_args=np.array(solve_args), manifest=np.array(rel),
+                        ane2=np.array([l for l in sol.splitlines() if l.startswith("ANE2:")]),
+                        head=np.array([l for l in out.splitlines() if l.startswith("LLR2D")]), **cfgs)
+
+
 def main():
+    """Writes every fixture, or only the groups named on the command line
+    (loops, traj, cli, replicas, llr, files)."""
     if not os.path.exists(os.path.join(REFBIN, "ref_harness")):
         sys.exit("build the reference first: make -f oracle/ref.mk")
+    only = set(sys.argv[1:])
+    want = lambda k: not only or k in only
     g = "mys1080-v4"
-    loops_fixture("s1080_2x2", g, 4, 2, 2, 11)
-    loops_fixture("s1080_4x4", g, 4, 4, 4, 12)
-    loops_fixture("s1080_6x4", g, 4, 6, 4, 13)
-    loops_fixture("s216_4x6", "S216", 4, 4, 6, 14)
-    loops_fixture("mybi_3x3", "myBI", 4, 3, 3, 15)
-    loops_fixture("s108_d3_6x4", "S108", 3, 6, 4, 16)
-    # trajectories of the reference update() at one thread
-    traj_fixture("c1", g, 4, 2, 2, 0.0, 1.5, 0.1, 1, 110, every=10)
-    traj_fixture("s1080_4x4_b2", g, 4, 4, 4, 0.0, 3.0, -0.25, 5, 6)
-    traj_fixture("s1080_4x4_wilson", g, 4, 4, 4, 0.0, 2.0, 0.0, 3, 6)
-    traj_fixture("s216_4x6", "S216", 4, 4, 6, 0.0, 2.5, -0.2, 21, 3)
-    traj_fixture("mybi_3x3", "myBI", 4, 3, 3, 0.0, 1.0, 0.5, 77, 4)
-    # full CLI runs (config 1; statistics runs)
-    cli_fixture("c1", [os.path.join(REF, "groups", g), "4", "2", "2", "0", "1.5", "0.1", "1", "10", "100", "100"])
-    # long single chains where the Metropolis chain mixes (acceptance 0.2-0.4)
-    cli_fixture("stats_4x4", [os.path.join(REF, "groups", g), "4", "4", "4", "0", "1.2", "-0.1", "9", "2", "1000",
-                              "50"])
-    cli_fixture("stats_6x6", [os.path.join(REF, "groups", g), "4", "6", "6", "0", "1.6", "-0.25", "19", "1", "600",
-                              "50"])
-    # config 1 is frozen (acceptance 0.002): compare ensembles of independent replicas
-    replicas_fixture("c1", g, ["4", "2", "2", "0", "1.5", "0.1"], range(1000, 1064), ["10", "10", "100"], 2, 2)
-    # llr.cpp: ladders of windows on axis 1 and 2 (exact-output fixtures), and
-    # one window solved 16 times (8 repeats x hot/cold) for a statistical check
-    llr_fixture("axis1_3x3", [g, "4", "3", "3", "1", "-0.1", "-500", "-800", "150", "60", "-1.0", "5", "3", "2",
-                              "1"])
-    llr_fixture("axis2_4x3", [g, "4", "4", "3", "2", "1.2", "-300", "-500", "100", "50", "0.5", "6", "2", "2",
-                              "1"])
-    llr_fixture("s216_axis1", ["S216", "4", "3", "4", "1", "0.0", "-900", "-1000", "100", "60", "1.0", "8", "2",
-                               "3", "1"])
-    llr_fixture("stats_4x4", [g, "4", "4", "4", "1", "-0.1", "-1500", "-1500", "100", "75", "1.5", "20", "20",
-                              "30", "8"])
+    if want("loops"):
+        loops_fixture("s1080_2x2", g, 4, 2, 2, 11)
+        loops_fixture("s1080_4x4", g, 4, 4, 4, 12)
+        loops_fixture("s1080_6x4", g, 4, 6, 4, 13)
+        loops_fixture("s216_4x6", "S216", 4, 4, 6, 14)
+        loops_fixture("mybi_3x3", "myBI", 4, 3, 3, 15)
+        loops_fixture("s108_d3_6x4", "S108", 3, 6, 4, 16)
+    if want("traj"):  # trajectories of the reference update() at one thread
+        traj_fixture("c1", g, 4, 2, 2, 0.0, 1.5, 0.1, 1, 110, every=10)
+        traj_fixture("s1080_4x4_b2", g, 4, 4, 4, 0.0, 3.0, -0.25, 5, 6)
+        traj_fixture("s1080_4x4_wilson", g, 4, 4, 4, 0.0, 2.0, 0.0, 3, 6)
+        traj_fixture("s216_4x6", "S216", 4, 4, 6, 0.0, 2.5, -0.2, 21, 3)
+        traj_fixture("mybi_3x3", "myBI", 4, 3, 3, 0.0, 1.0, 0.5, 77, 4)
+    if want("cli"):  # full CLI runs (config 1; long chains where the Metropolis chain mixes)
+        cli_fixture("c1", [os.path.join(REF, "groups", g), "4", "2", "2", "0", "1.5", "0.1", "1", "10", "100", "100"])
+        cli_fixture("stats_4x4", [os.path.join(REF, "groups", g), "4", "4", "4", "0", "1.2", "-0.1", "9", "2",
+                                  "1000", "50"])
+        cli_fixture("stats_6x6", [os.path.join(REF, "groups", g), "4", "6", "6", "0", "1.6", "-0.25", "19", "1",
+                                  "600", "50"])
+    if want("replicas"):  # config 1 is frozen (acceptance 0.002): ensembles of independent replicas
+        replicas_fixture("c1", g, ["4", "2", "2", "0", "1.5", "0.1"], range(1000, 1064), ["10", "10", "100"], 2, 2)
+    if want("llr"):
+        # llr.cpp: ladders of windows on axis 1 and 2 (exact-output fixtures),
+        # and one window solved 16 times (8 repeats x hot/cold)
+        llr_fixture("axis1_3x3", [g, "4", "3", "3", "1", "-0.1", "-500", "-800", "150", "60", "-1.0", "5", "3", "2",
+                                  "1"])
+        llr_fixture("axis2_4x3", [g, "4", "4", "3", "2", "1.2", "-300", "-500", "100", "50", "0.5", "6", "2", "2",
+                                  "1"])
+        llr_fixture("s216_axis1", ["S216", "4", "3", "4", "1", "0.0", "-900", "-1000", "100", "60", "1.0", "8", "2",
+                                   "3", "1"])
+        llr_fixture("stats_4x4", [g, "4", "4", "4", "1", "-0.1", "-1500", "-1500", "100", "75", "1.5", "20", "20",
+                                  "30", "8"])
+    if want("files"):  # configuration files: savecfg's SU(3) output on config 1, llr2d's cells
+        kentucky_fixture("c1", g, ["4", "2", "2", "0", "1.5", "0.1"], 1, 3, 2)
+        llr2d_fixture("seed_3x3", g, 4, 3, 3,
+                      ["-400", "-600", "100", "40", "0", "0.8", "0", "60", "40", "1", "1.0", "5"],
+                      ["1.0", "10", "10", "7"])
     meta = {
         "generator": "tests/golden/make_golden.py",
         "reference_build": "oracle/ref.mk (g++ -O3 -fopenmp -std=c++11, as the reference Makefile)",
