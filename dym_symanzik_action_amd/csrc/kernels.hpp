@@ -110,10 +110,12 @@ struct LlrChain {
   long long *dE2;    // per link: proposed change of S2
   uint8_t *nacc;     // per link: accepted hits of its proposal
   const PassRange *passes;
-  int axis;          // 1: window S1 (plaquettes), 2: window S2 (rectangles)
-  double bg;         // coupling of the other action
-  long long lo, hi;  // window [Ewin - hw, Ewin + hw] in fixed point
+  int axis;          // 1: window S1 (plaquettes), 2: window S2 (rectangles), 3: box on both
+  double bg;         // coupling of the other action (axes 1, 2)
+  long long lo, hi;  // window [Ewin - hw, Ewin + hw] in fixed point (axis 3: on S1)
   double Ewin, W;    // centre and width 2*hw (Robbins-Monro)
+  long long lo2, hi2;  // axis 3: the S2 window
+  double Ewin2, W2;
   uint32_t K, NRM;   // constrained sweeps per RM iteration; iteration cap
   double tol;        // stop once |delta a| < tol (0: run NRM iterations)
   uint32_t cap;      // drive-in sweep cap
@@ -123,8 +125,9 @@ struct LlrChain {
 struct LlrState {
   long long S1, S2;  // running plaquette / rectangle actions (fixed point)
   long long r1, r2;  // resync accumulators
-  double a;          // tilt a_n
-  double sumE;       // sum of the windowed energy over the iteration's sweeps
+  double a;          // tilt a_n (axis 3: of S1)
+  double sumE;       // sum of the windowed energy over the iteration's sweeps (axis 3: S1)
+  double a2, sumE2;  // axis 3: tilt of S2 and its sum
   double last_da;    // |delta a| of the last RM step
   double maxdrift;   // max |incremental - recomputed| energy
   uint32_t phase, k, m, seed_sweeps, drive, resync, reached, pad;

@@ -38,21 +38,57 @@ __device__ inline long long readlane64(long long v, uint32_t k) {
 
 __device__ inline bool llr_active(uint32_t phase) { return phase == kLlrSeed || phase == kLlrRM; }
 
-// drive-in: is the windowed energy outside the window at the start of a pass?
+__device__ inline bool llr_inside(const LlrChain &LC, long long E1, long long E2) {
+  if (LC.axis == 3) return E1 >= LC.lo && E1 <= LC.hi && E2 >= LC.lo2 && E2 <= LC.hi2;
+  const long long Ew = LC.axis == 1 ? E1 : E2;
+  return Ew >= LC.lo && Ew <= LC.hi;
+}
+
+// drive-in: is the energy outside the window (box) at the start of a pass?
 __device__ inline bool llr_drive(const LlrChain &LC, const LlrState &S) {
-  const long long Ew = LC.axis == 1 ? S.S1 : S.S2;
-  return S.phase == kLlrSeed && (Ew < LC.lo || Ew > LC.hi);
+  return S.phase == kLlrSeed && !llr_inside(LC, S.S1, S.S2);
 }
 
 __device__ inline long long llabs_(long long v) { return v < 0 ? -v : v; }
 
-// Acceptance of a link's change of the windowed energy from Eb to Ea.
-__device__ inline bool llr_rule(bool drive, long long lo, long long hi, long long Eb, long long Ea) {
-  if (drive) {
-    const long long mid = lo + (hi - lo) / 2;
-    return llabs_(Ea - mid) < llabs_(Eb - mid);
+// The acceptance rule of one pass: windowed energies (E1, E2) before (b) and
+// after (a) a link's proposal.  Axes 1, 2: inside the window, or during
+// drive-in closer to its centre (llr.cpp:203-212).  Axis 3: inside the box,
+// or during drive-in closer to the cell centre in units of the half widths
+// (squared distances; llr2d.cpp:200-213 compares hypot).
+struct LlrRule {
+  int axis;
+  bool drive;
+  long long lo, hi, lo2, hi2, mid, mid2;
+  double hw, hw2;
+  __device__ double dist2(long long E1, long long E2) const {
+    const double x = from_fix(E1 - mid) / hw, y = from_fix(E2 - mid2) / hw2;
+    return x * x + y * y;
   }
-  return Ea >= lo && Ea <= hi;
+  __device__ bool operator()(long long E1b, long long E2b, long long E1a, long long E2a) const {
+    if (axis == 3) {
+      if (drive) return dist2(E1a, E2a) < dist2(E1b, E2b);
+      return E1a >= lo && E1a <= hi && E2a >= lo2 && E2a <= hi2;
+    }
+    const long long Eb = axis == 1 ? E1b : E2b, Ea = axis == 1 ? E1a : E2a;
+    if (drive) return llabs_(Ea - mid) < llabs_(Eb - mid);
+    return Ea >= lo && Ea <= hi;
+  }
+};
+
+__device__ inline LlrRule llr_rule_of(const LlrChain &LC, const LlrState &S) {
+  LlrRule R;
+  R.axis = LC.axis;
+  R.drive = llr_drive(LC, S);
+  R.lo = LC.lo;
+  R.hi = LC.hi;
+  R.lo2 = LC.lo2;
+  R.hi2 = LC.hi2;
+  R.mid = LC.lo + (LC.hi - LC.lo) / 2;
+  R.mid2 = LC.lo2 + (LC.hi2 - LC.lo2) / 2;
+  R.hw = 0.5 * LC.W;
+  R.hw2 = 0.5 * LC.W2;
+  return R;
 }
 
 template <int D>
@@ -74,9 +110,13 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
   const TabView T{BC.T.mult, s_inv, P};
   const Geom &G = BC.G;
   const int axis = LC.axis;
-  const bool drive = llr_drive(LC, S);
+  const LlrRule R = llr_rule_of(LC, S);
+  const bool drive = R.drive;
+  // drive-in direction: axes 1, 2 toward the window; axis 3 down the gradient
+  // of the squared distance to the cell centre at the start of the pass
   const double sign = from_fix(axis == 1 ? S.S1 : S.S2) < LC.Ewin ? 1.0 : -1.0;
-  const double a = S.a, bg = LC.bg;
+  const double gx = from_fix(S.S1 - R.mid) / R.hw / R.hw, gy = from_fix(S.S2 - R.mid2) / R.hw2 / R.hw2;
+  const double a = S.a, a2 = S.a2, bg = LC.bg;
   const uint32_t sw_lo = (uint32_t)S.sweep, sw_hi = (uint32_t)(S.sweep >> 32);
   const double npool_d = (double)BC.T.npool;
   for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x) {
@@ -115,13 +155,19 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
 #pragma unroll
       for (int q = 0; q < NR; ++q) m1n += s_retr[T.M(bnew, re[q])];
       const double dS1 = r1n - r1, dS2 = m1n - m1;
-      const double dEw = axis == 1 ? dS1 : dS2;
-      const double dEbg = axis == 1 ? dS2 : dS1;
       bool accept;
-      if (drive) {
-        accept = sign * dEw > 0.0;  // greedy move toward the window (llr.cpp:208-209)
+      if (axis == 3) {
+        if (drive)
+          accept = gx * dS1 + gy * dS2 < 0.0;  // greedy move toward the cell centre
+        else
+          accept = dsa_exp(-a * dS1 - a2 * dS2) > u2;  // llr2d.cpp:211-212
       } else {
-        accept = dsa_exp(-a * dEw - bg * dEbg) > u2;  // llr.cpp:211-212
+        const double dEw = axis == 1 ? dS1 : dS2;
+        const double dEbg = axis == 1 ? dS2 : dS1;
+        if (drive)
+          accept = sign * dEw > 0.0;  // greedy move toward the window (llr.cpp:208-209)
+        else
+          accept = dsa_exp(-a * dEw - bg * dEbg) > u2;  // llr.cpp:211-212
       }
       if (accept) {
         b = bnew;
@@ -174,9 +220,7 @@ __global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrSt
   LlrState &S = st[c];
   if (!llr_active(S.phase) || pass >= LC.npass) return;
   const PassRange pr = LC.passes[pass];
-  const bool drive = llr_drive(LC, S);
-  const bool ax1 = LC.axis == 1;
-  const long long lo = LC.lo, hi = LC.hi;
+  const LlrRule R = llr_rule_of(LC, S);
   long long E1 = S.S1, E2 = S.S2;
   unsigned long long acc = 0;
   const Geom &G = LC.base.G;
@@ -196,14 +240,12 @@ __global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrSt
       const uint32_t base = slab + 64u * q;
       if (base >= pr.count) break;
       const uint32_t n = pr.count - base < 64 ? pr.count - base : 64;
-      const long long dw = ax1 ? r1[q] : r2[q];
       const long long P1 = scan_i64(r1[q], lane), P2 = scan_i64(r2[q], lane);
       const long long PN = scan_i64((long long)rn[q], lane);
-      const long long Pw = ax1 ? P1 : P2;
       for (uint32_t s0 = 0; s0 < n;) {  // s0 is uniform over the wave
-        const long long Ew = ax1 ? E1 : E2;
-        const long long Enew = Ew + Pw - (s0 ? readlane64(Pw, s0 - 1) : 0);
-        const bool ok = (uint32_t)lane < s0 || (uint32_t)lane >= n || llr_rule(drive, lo, hi, Enew - dw, Enew);
+        const long long N1 = E1 + P1 - (s0 ? readlane64(P1, s0 - 1) : 0);
+        const long long N2 = E2 + P2 - (s0 ? readlane64(P2, s0 - 1) : 0);
+        const bool ok = (uint32_t)lane < s0 || (uint32_t)lane >= n || R(N1 - r1[q], N2 - r2[q], N1, N2);
         const unsigned long long bad = __ballot(!ok);
         const uint32_t f = bad ? (uint32_t)__builtin_ctzll(bad) : n;
         if (f > s0) {  // accept [s0, f)
@@ -237,15 +279,16 @@ __global__ void k_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n
   if (!llr_active(S.phase)) return;
   S.sweep += 1;
   S.hit += (unsigned long long)LC.base.G.V * (unsigned long long)LC.base.G.D * kNHit;
-  const long long Ew = LC.axis == 1 ? S.S1 : S.S2;
+  const long long Ew = LC.axis == 2 ? S.S2 : S.S1;  // the S1 window for axes 1 and 3
   if (S.phase == kLlrSeed) {
     S.seed_sweeps += 1;
-    if (Ew >= LC.lo && Ew <= LC.hi) {
+    if (llr_inside(LC, S.S1, S.S2)) {
       S.phase = kLlrRM;
       S.reached = 1;
       S.k = 0;
       S.m = 0;
       S.sumE = 0;
+      S.sumE2 = 0;
       S.resync = 1;
     } else if (S.seed_sweeps >= LC.cap) {
       S.phase = kLlrDone;
@@ -254,15 +297,22 @@ __global__ void k_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n
     return;
   }
   S.sumE += from_fix(Ew);
+  if (LC.axis == 3) S.sumE2 += from_fix(S.S2);
   S.k += 1;
   if (S.k == LC.K) {
-    const double a_old = S.a;
+    const double a_old = S.a, a2_old = S.a2;
     const double g = S.sumE / LC.K - LC.Ewin;
     S.a += (12.0 / (LC.W * LC.W * (S.m + 1))) * g;
+    S.last_da = fabs(S.a - a_old);
+    if (LC.axis == 3) {  // llr2d.cpp:247-248
+      const double g2 = S.sumE2 / LC.K - LC.Ewin2;
+      S.a2 += (12.0 / (LC.W2 * LC.W2 * (S.m + 1))) * g2;
+      S.last_da = fmax(S.last_da, fabs(S.a2 - a2_old));
+      S.sumE2 = 0;
+    }
     S.m += 1;
     S.k = 0;
     S.sumE = 0;
-    S.last_da = fabs(S.a - a_old);
     S.resync = 1;
     if ((LC.tol > 0.0 && S.last_da < LC.tol) || S.m >= LC.NRM) S.phase = kLlrDone;
   }
@@ -310,8 +360,9 @@ __global__ void k_llr_resync_apply(const LlrChain *chains, LlrState *st, uint32_
   if (c >= n || !st[c].resync) return;
   LlrState &S = st[c];
   if (S.resync == 1) {
-    const bool ax1 = chains[c].axis == 1;
-    const double drift = fabs(from_fix(ax1 ? S.S1 : S.S2) - from_fix(ax1 ? S.r1 : S.r2));
+    const int axis = chains[c].axis;
+    const double d1 = fabs(from_fix(S.S1) - from_fix(S.r1)), d2 = fabs(from_fix(S.S2) - from_fix(S.r2));
+    const double drift = axis == 1 ? d1 : axis == 2 ? d2 : fmax(d1, d2);
     if (drift > S.maxdrift) S.maxdrift = drift;
   }
   S.S1 = S.r1;

@@ -209,8 +209,9 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   if ((int)p->D != L->D) return fail(DSA_E_ARG, "chain dimension differs from the engine's");
   dsa_chain_params cp{p->D, p->Nt, p->Nx, 0, 0, 0, p->seed, p->start, DSA_RNG_PHILOX, DSA_ORDER_COLOUR};
   if (int rc = dsa::check_params(&cp)) return rc;
-  if (p->axis != 1 && p->axis != 2) return fail(DSA_E_ARG, "axis must be 1 or 2");
+  if (p->axis != 1 && p->axis != 2 && p->axis != 3) return fail(DSA_E_ARG, "axis must be 1, 2 or 3");
   if (!(p->hw > 0) || !p->K || !p->NRM) return fail(DSA_E_ARG, "need hw > 0, K > 0, NRM > 0");
+  if (p->axis == 3 && !(p->hw2 > 0)) return fail(DSA_E_ARG, "axis 3 needs hw2 > 0");
   const uint64_t bytes = footprint(*p);
   if (L->budget && L->used_bytes + bytes > L->budget) return fail(DSA_E_RANGE, "HBM budget exhausted");
   uint32_t c = 0;
@@ -282,6 +283,14 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   d.hi = dsa::to_fix(p->Ewin + p->hw);
   d.Ewin = p->Ewin;
   d.W = 2 * p->hw;
+  if (p->axis == 3) {
+    d.lo2 = dsa::to_fix(p->Ewin2 - p->hw2);
+    d.hi2 = dsa::to_fix(p->Ewin2 + p->hw2);
+    d.Ewin2 = p->Ewin2;
+    d.W2 = 2 * p->hw2;
+  } else {
+    d.W2 = 1.0;  // unused
+  }
   d.K = p->K;
   d.NRM = p->NRM;
   d.tol = p->tol;
@@ -289,6 +298,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   d.npass = (uint32_t)s.S.passes.size();
   dsa::LlrState st{};
   st.a = p->a0;
+  st.a2 = p->axis == 3 ? p->a20 : 0.0;
   st.phase = dsa::kLlrSeed;
   st.resync = 2;  // initial energies: computed, not compared
   if (hipMemcpy(L->d_chains + c, &d, sizeof d, hipMemcpyHostToDevice) != hipSuccess ||
@@ -340,6 +350,7 @@ int dsa_llr_status_get(dsa_llr *L, uint32_t slot, dsa_llr_status *out) {
   out->sweeps = s.sweep;
   out->hit = s.hit;
   out->acc = s.acc;
+  out->a2 = s.a2;
   return 0;
 }
 

@@ -435,6 +435,7 @@ class LlrParams(C.Structure):
         ("axis", C.c_int32), ("bg", C.c_double), ("Ewin", C.c_double), ("hw", C.c_double), ("a0", C.c_double),
         ("seed", C.c_int32), ("start", C.c_int32), ("K", C.c_uint32), ("NRM", C.c_uint32),
         ("tol", C.c_double), ("cap", C.c_uint32),
+        ("Ewin2", C.c_double), ("hw2", C.c_double), ("a20", C.c_double),
     ]
 
 
@@ -443,7 +444,7 @@ class LlrStatus(C.Structure):
         ("a", C.c_double), ("S1", C.c_double), ("S2", C.c_double), ("last_da", C.c_double),
         ("maxdrift", C.c_double), ("phase", C.c_uint32), ("iterations", C.c_uint32),
         ("seed_sweeps", C.c_uint32), ("reached", C.c_uint32), ("sweeps", C.c_uint64), ("hit", C.c_uint64),
-        ("acc", C.c_uint64),
+        ("acc", C.c_uint64), ("a2", C.c_double),
     ]
 
     def as_dict(self):
@@ -453,8 +454,11 @@ class LlrStatus(C.Structure):
 LLR_PHASE = {0: "seed", 1: "rm", 2: "done", 3: "idle"}
 
 
-def llr_params(D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, start="hot", K=20, NRM=50, tol=0.0, cap=0):
-    return LlrParams(D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, DSA_START[start], K, NRM, tol, cap)
+def llr_params(D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, start="hot", K=20, NRM=50, tol=0.0, cap=0, Ewin2=0.0,
+               hw2=0.0, a20=0.0):
+    """axis 1/2: one window (llr.cpp); axis 3: the box (Ewin, hw) x (Ewin2, hw2)
+    on (S1, S2) with tilts (a0, a20) (llr2d.cpp); bg is unused there."""
+    return LlrParams(D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, DSA_START[start], K, NRM, tol, cap, Ewin2, hw2, a20)
 
 
 class LlrEngine:

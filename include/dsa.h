@@ -190,7 +190,8 @@ int dsa_batch_last_ms(dsa_batch *b, double *ms);
  * engine (one D per engine) and are advanced together. */
 typedef struct {
   uint32_t D, Nt, Nx;
-  int32_t axis;       /* 1: window S1 (llr.cpp AXIS=1), 2: window S2 */
+  int32_t axis;       /* 1: window S1 (llr.cpp AXIS=1), 2: window S2,
+                         3: box on (S1, S2) with tilts (a0, a20) (llr2d.cpp) */
   double bg;          /* coupling of the other action (llr.cpp BG) */
   double Ewin, hw;    /* window centre and half width */
   double a0;          /* initial tilt */
@@ -199,6 +200,10 @@ typedef struct {
   uint32_t K, NRM;    /* constrained sweeps per RM iteration; iteration cap */
   double tol;         /* stop once |delta a| < tol; 0 runs NRM iterations */
   uint32_t cap;       /* drive-in sweep cap (0: 20000, llr.cpp:225) */
+  /* axis 3 only: the S2 window and its initial tilt; Ewin, hw, a0 are the S1
+   * ones (llr2d.cpp:180-259: drive-in greedy on the distance to the cell
+   * centre in units of the half widths, RM on both tilts) */
+  double Ewin2, hw2, a20;
 } dsa_llr_params;
 
 enum { DSA_LLR_SEED = 0, DSA_LLR_RM = 1, DSA_LLR_DONE = 2, DSA_LLR_IDLE = 3 };
@@ -211,6 +216,7 @@ typedef struct {
   uint32_t phase;     /* DSA_LLR_* */
   uint32_t iterations, seed_sweeps, reached;
   uint64_t sweeps, hit, acc;
+  double a2;          /* axis 3: the S2 tilt */
 } dsa_llr_status;
 
 typedef struct dsa_llr dsa_llr;

@@ -119,9 +119,11 @@ int ogp_randint(uint64_t *x, int lo, int hi) { return og_randint(x, lo, hi); }
 /* ---- LLR ---- */
 #include "oracle_llr.h"
 void *ogp_llr_new(void *g, unsigned D, unsigned Nt, unsigned Nx, int axis, double bg, double Ewin, double hw,
-                  double a0, int seed, int cold, unsigned K, unsigned NRM, double tol, unsigned cap) {
+                  double a0, int seed, int cold, unsigned K, unsigned NRM, double tol, unsigned cap, double Ewin2,
+                  double hw2, double a20) {
   ogl_dev *L = (ogl_dev *)calloc(1, sizeof *L);
-  if (ogl_dev_init(L, (og_group *)g, D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, cold, K, NRM, tol, cap)) {
+  if (ogl_dev_init(L, (og_group *)g, D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, cold, K, NRM, tol, cap, Ewin2, hw2,
+                   a20)) {
     free(L);
     return NULL;
   }
@@ -129,12 +131,19 @@ void *ogp_llr_new(void *g, unsigned D, unsigned Nt, unsigned Nx, int axis, doubl
 }
 void ogp_llr_free(void *L) { if (L) { ogl_dev_free((ogl_dev *)L); free(L); } }
 void ogp_llr_step(void *L, unsigned n) { ogl_dev_step((ogl_dev *)L, n); }
-/* out: a, S1, S2, last_da, maxdrift, phase, m, seed_sweeps, reached, sweep, hit, acc */
+/* start from a given configuration (energies recomputed, not compared) */
+void ogp_llr_set_links(void *Lv, const int32_t *in) {
+  ogl_dev *L = (ogl_dev *)Lv;
+  memcpy(L->c.L.a, in, (size_t)L->c.L.V * L->c.L.D * 4);
+  L->resync = 2;
+}
+/* out: a, S1, S2, last_da, maxdrift, phase, m, seed_sweeps, reached, sweep, hit, acc, a2 */
 void ogp_llr_status(void *Lv, double *out) {
   ogl_dev *L = (ogl_dev *)Lv;
   out[0] = L->a; out[1] = (double)L->S1 / 4294967296.0; out[2] = (double)L->S2 / 4294967296.0;
   out[3] = L->last_da; out[4] = L->maxdrift; out[5] = L->phase; out[6] = L->m; out[7] = L->seed_sweeps;
   out[8] = L->reached; out[9] = (double)L->sweep; out[10] = (double)L->hit; out[11] = (double)L->acc;
+  out[12] = L->a2;
 }
 void ogp_llr_links(void *Lv, int32_t *out) {
   ogl_dev *L = (ogl_dev *)Lv;

@@ -234,7 +234,7 @@ static double ogl_unfix(long long v) { return (double)v / 4294967296.0; }
 
 int ogl_dev_init(ogl_dev *L, const og_group *g, unsigned D, unsigned Nt, unsigned Nx, int axis, double bg,
                  double Ewin, double hw, double a0, int seed, int cold, unsigned K, unsigned NRM, double tol,
-                 unsigned cap) {
+                 unsigned cap, double Ewin2, double hw2, double a20) {
   memset(L, 0, sizeof *L);
   if (og_chain_init(&L->c, g, D, Nt, Nx, 0, 0, 0, seed, cold, OG_RNG_PHILOX, OG_ORDER_COLOUR)) return -1;
   L->axis = axis;
@@ -243,6 +243,14 @@ int ogl_dev_init(ogl_dev *L, const og_group *g, unsigned D, unsigned Nt, unsigne
   L->W = 2 * hw;
   L->lo = ogl_fix(Ewin - hw);
   L->hi = ogl_fix(Ewin + hw);
+  L->W2 = 1.0;
+  if (axis == 3) {
+    L->Ewin2 = Ewin2;
+    L->W2 = 2 * hw2;
+    L->lo2 = ogl_fix(Ewin2 - hw2);
+    L->hi2 = ogl_fix(Ewin2 + hw2);
+    L->a2 = a20;
+  }
   L->K = K;
   L->NRM = NRM;
   L->tol = tol;
@@ -278,9 +286,9 @@ static void dev_resync(ogl_dev *L) {
       for (unsigned d2 = 0; d2 < D; ++d2)
         if (d1 != d2) r2 += ogl_fix(g->retr[og_wilson(Lt, i, d1, d2, 2, 1)]);
   }
-  const int ax1 = L->axis == 1;
   if (L->resync == 1) {
-    double drift = fabs(ogl_unfix(ax1 ? L->S1 : L->S2) - ogl_unfix(ax1 ? r1 : r2));
+    const double d1 = fabs(ogl_unfix(L->S1) - ogl_unfix(r1)), d2 = fabs(ogl_unfix(L->S2) - ogl_unfix(r2));
+    const double drift = L->axis == 1 ? d1 : L->axis == 2 ? d2 : fmax(d1, d2);
     if (drift > L->maxdrift) L->maxdrift = drift;
   }
   L->S1 = r1;
@@ -290,12 +298,47 @@ static void dev_resync(ogl_dev *L) {
 
 static long long llabs_(long long v) { return v < 0 ? -v : v; }
 
-static int dev_rule(int drive, long long lo, long long hi, long long Eb, long long Ea) {
-  if (drive) {
-    const long long mid = lo + (hi - lo) / 2;
-    return llabs_(Ea - mid) < llabs_(Eb - mid);
+static int dev_inside(const ogl_dev *L, long long E1, long long E2) {
+  if (L->axis == 3) return E1 >= L->lo && E1 <= L->hi && E2 >= L->lo2 && E2 <= L->hi2;
+  const long long Ew = L->axis == 1 ? E1 : E2;
+  return Ew >= L->lo && Ew <= L->hi;
+}
+
+/* the pass's acceptance rule on the running energies before (b) / after (a) */
+typedef struct {
+  int axis, drive;
+  long long lo, hi, lo2, hi2, mid, mid2;
+  double hw, hw2;
+} dev_rule_t;
+
+static dev_rule_t dev_rule_of(const ogl_dev *L, int drive) {
+  dev_rule_t R;
+  R.axis = L->axis;
+  R.drive = drive;
+  R.lo = L->lo;
+  R.hi = L->hi;
+  R.lo2 = L->lo2;
+  R.hi2 = L->hi2;
+  R.mid = L->lo + (L->hi - L->lo) / 2;
+  R.mid2 = L->lo2 + (L->hi2 - L->lo2) / 2;
+  R.hw = 0.5 * L->W;
+  R.hw2 = 0.5 * L->W2;
+  return R;
+}
+
+static double dev_dist2(const dev_rule_t *R, long long E1, long long E2) {
+  const double x = ogl_unfix(E1 - R->mid) / R->hw, y = ogl_unfix(E2 - R->mid2) / R->hw2;
+  return x * x + y * y;
+}
+
+static int dev_rule(const dev_rule_t *R, long long E1b, long long E2b, long long E1a, long long E2a) {
+  if (R->axis == 3) {
+    if (R->drive) return dev_dist2(R, E1a, E2a) < dev_dist2(R, E1b, E2b);
+    return E1a >= R->lo && E1a <= R->hi && E2a >= R->lo2 && E2a <= R->hi2;
   }
-  return Ea >= lo && Ea <= hi;
+  const long long Eb = R->axis == 1 ? E1b : E2b, Ea = R->axis == 1 ? E1a : E2a;
+  if (R->drive) return llabs_(Ea - R->mid) < llabs_(Eb - R->mid);
+  return Ea >= R->lo && Ea <= R->hi;
 }
 
 /* One colour pass: every link proposes (NHIT tilted or greedy hits), then the
@@ -307,8 +350,10 @@ static void dev_pass(ogl_dev *L, unsigned mu, unsigned colour) {
   const unsigned D = Lt->D, V = Lt->V;
   const int ns = 2 * (int)(D - 1), nr = 6 * (int)(D - 1);
   const long long Ew = L->axis == 1 ? L->S1 : L->S2;
-  const int drive = L->phase == OGL_SEED && (Ew < L->lo || Ew > L->hi);
+  const int drive = L->phase == OGL_SEED && !dev_inside(L, L->S1, L->S2);
+  const dev_rule_t R = dev_rule_of(L, drive);
   const double sign = ogl_unfix(Ew) < L->Ewin ? 1.0 : -1.0;
+  const double gx = ogl_unfix(L->S1 - R.mid) / R.hw / R.hw, gy = ogl_unfix(L->S2 - R.mid2) / R.hw2 / R.hw2;
   const uint32_t key[2] = {(uint32_t)L->c.seed, OG_KEY_LLR};
   const uint32_t swl = (uint32_t)L->sweep, swh = (uint32_t)(L->sweep >> 32);
   const uint8_t *col = L->c.col + (size_t)mu * V;
@@ -343,12 +388,19 @@ static void dev_pass(ogl_dev *L, unsigned mu, unsigned colour) {
       for (int j = 0; j < ns; ++j) r1n += g->retr[MUL(bnew, st[j])];
       for (int k = 0; k < nr; ++k) m1n += g->retr[MUL(bnew, re[k])];
       const double dS1 = r1n - r1, dS2 = m1n - m1;
-      const double dEw = L->axis == 1 ? dS1 : dS2, dEbg = L->axis == 1 ? dS2 : dS1;
       int accept;
-      if (drive)
-        accept = sign * dEw > 0.0;
-      else
-        accept = og_exp(-L->a * dEw - L->bg * dEbg) > u2;
+      if (L->axis == 3) {
+        if (drive)
+          accept = gx * dS1 + gy * dS2 < 0.0;
+        else
+          accept = og_exp(-L->a * dS1 - L->a2 * dS2) > u2;
+      } else {
+        const double dEw = L->axis == 1 ? dS1 : dS2, dEbg = L->axis == 1 ? dS2 : dS1;
+        if (drive)
+          accept = sign * dEw > 0.0;
+        else
+          accept = og_exp(-L->a * dEw - L->bg * dEbg) > u2;
+      }
       if (accept) {
         b = bnew;
         r1 = r1n;
@@ -375,9 +427,7 @@ static void dev_pass(ogl_dev *L, unsigned mu, unsigned colour) {
   unsigned q = 0;
   for (unsigned i = 0; i < V; ++i) {
     if (col[i] != colour) continue;
-    const long long Ec = L->axis == 1 ? L->S1 : L->S2;
-    const long long dw = L->axis == 1 ? L->d1[q] : L->d2[q];
-    if (dev_rule(drive, L->lo, L->hi, Ec, Ec + dw)) {
+    if (dev_rule(&R, L->S1, L->S2, L->S1 + L->d1[q], L->S2 + L->d2[q])) {
       L->S1 += L->d1[q];
       L->S2 += L->d2[q];
       L->acc += L->na[q];
@@ -391,15 +441,16 @@ static void dev_pass(ogl_dev *L, unsigned mu, unsigned colour) {
 static void dev_sweep_end(ogl_dev *L) {
   L->sweep++;
   L->hit += (uint64_t)L->c.L.V * L->c.L.D * OG_NHIT;
-  const long long Ew = L->axis == 1 ? L->S1 : L->S2;
+  const long long Ew = L->axis == 2 ? L->S2 : L->S1; /* the S1 window for axes 1 and 3 */
   if (L->phase == OGL_SEED) {
     L->seed_sweeps++;
-    if (Ew >= L->lo && Ew <= L->hi) {
+    if (dev_inside(L, L->S1, L->S2)) {
       L->phase = OGL_RM;
       L->reached = 1;
       L->k = 0;
       L->m = 0;
       L->sumE = 0;
+      L->sumE2 = 0;
       L->resync = 1;
     } else if (L->seed_sweeps >= L->cap) {
       L->phase = OGL_DONE;
@@ -408,15 +459,22 @@ static void dev_sweep_end(ogl_dev *L) {
     return;
   }
   L->sumE += ogl_unfix(Ew);
+  if (L->axis == 3) L->sumE2 += ogl_unfix(L->S2);
   L->k++;
   if (L->k == L->K) {
-    const double a_old = L->a;
+    const double a_old = L->a, a2_old = L->a2;
     const double gr = L->sumE / L->K - L->Ewin;
     L->a += (12.0 / (L->W * L->W * (L->m + 1))) * gr;
+    L->last_da = fabs(L->a - a_old);
+    if (L->axis == 3) { /* llr2d.cpp:247-248 */
+      const double g2 = L->sumE2 / L->K - L->Ewin2;
+      L->a2 += (12.0 / (L->W2 * L->W2 * (L->m + 1))) * g2;
+      L->last_da = fmax(L->last_da, fabs(L->a2 - a2_old));
+      L->sumE2 = 0;
+    }
     L->m++;
     L->k = 0;
     L->sumE = 0;
-    L->last_da = fabs(L->a - a_old);
     L->resync = 1;
     if ((L->tol > 0.0 && L->last_da < L->tol) || L->m >= L->NRM) L->phase = OGL_DONE;
   }

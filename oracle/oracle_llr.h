@@ -36,13 +36,16 @@ enum { OGL_SEED = 0, OGL_RM = 1, OGL_DONE = 2 };
 
 typedef struct {
   og_chain c;        /* lattice, colouring; Philox key = c.seed */
-  int axis;
+  int axis;          /* 1, 2: one window; 3: box on (S1, S2) (llr2d) */
   double bg, Ewin, W;
   long long lo, hi;
+  double Ewin2, W2;  /* axis 3: the S2 window */
+  long long lo2, hi2;
   unsigned K, NRM, cap;
   double tol;
   long long S1, S2;  /* fixed point, units 2^-32 */
   double a, sumE, last_da, maxdrift;
+  double a2, sumE2;  /* axis 3: tilt of S2 and its sum */
   unsigned phase, k, m, seed_sweeps, reached, resync;
   uint64_t sweep, acc, hit;
   int32_t *old;      /* scratch of one pass: previous values, */
@@ -52,7 +55,7 @@ typedef struct {
 
 int ogl_dev_init(ogl_dev *L, const og_group *g, unsigned D, unsigned Nt, unsigned Nx, int axis, double bg,
                  double Ewin, double hw, double a0, int seed, int cold, unsigned K, unsigned NRM, double tol,
-                 unsigned cap);
+                 unsigned cap, double Ewin2, double hw2, double a20);
 void ogl_dev_free(ogl_dev *L);
 /* n sweeps, with the resync bookkeeping the device does around them */
 void ogl_dev_step(ogl_dev *L, unsigned n);

@@ -207,12 +207,14 @@ class OLlr:
     """The device LLR algorithm restated on the CPU (oracle_llr.c, ogl_dev_*)."""
 
     def __init__(self, g: OGroup, D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, start="hot", K=20, NRM=50, tol=0.0,
-                 cap=0):
+                 cap=0, Ewin2=0.0, hw2=0.0, a20=0.0, links=None):
         L = lib()
         if not hasattr(L.ogp_llr_new, "_sig"):
             vp, u32, i32, dbl = C.c_void_p, C.c_uint32, C.c_int32, C.c_double
             L.ogp_llr_new.restype = vp
-            L.ogp_llr_new.argtypes = [vp, u32, u32, u32, i32, dbl, dbl, dbl, dbl, i32, i32, u32, u32, dbl, u32]
+            L.ogp_llr_new.argtypes = [vp, u32, u32, u32, i32, dbl, dbl, dbl, dbl, i32, i32, u32, u32, dbl, u32, dbl,
+                                      dbl, dbl]
+            L.ogp_llr_set_links.argtypes = [vp, vp]
             L.ogp_llr_free.argtypes = [vp]
             L.ogp_llr_step.argtypes = [vp, u32]
             L.ogp_llr_status.argtypes = [vp, vp]
@@ -220,20 +222,24 @@ class OLlr:
             L.ogp_llr_new._sig = True
         self.g = g
         self.h = L.ogp_llr_new(g.h, D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, 1 if start == "cold" else 0, K, NRM,
-                               tol, cap)
+                               tol, cap, Ewin2, hw2, a20)
         if not self.h:
             raise ValueError("oracle: llr init failed")
         self.V = Nt * Nx ** (D - 1)
         self.D = D
+        if links is not None:
+            arr = np.ascontiguousarray(links, dtype=np.int32)
+            assert arr.size == self.V * D
+            L.ogp_llr_set_links(self.h, _p(arr))
 
     def step(self, n=1):
         lib().ogp_llr_step(self.h, n)
 
     def status(self):
-        out = np.empty(12, np.float64)
+        out = np.empty(13, np.float64)
         lib().ogp_llr_status(self.h, _p(out))
         keys = ["a", "S1", "S2", "last_da", "maxdrift", "phase", "iterations", "seed_sweeps", "reached", "sweeps",
-                "hit", "acc"]
+                "hit", "acc", "a2"]
         return dict(zip(keys, out.tolist()))
 
     @property

@@ -140,11 +140,21 @@ def llr2d_fixture(name, group, D, Nt, Nx, seed_args, solve_args):
         for l in cells:
             path = l.split()[-1]
             cfgs[os.path.basename(path)[:-4]] = np.fromfile(path, np.int32)
+        # the same cells solved again with eight seeds (K = 20, NRM = 30): the
+        # spread of the reference's a_n per cell, for statistical checks
+        a1s, a2s = [], []
+        for sd in range(100, 108):
+            sol2 = run([os.path.join(REFBIN, "llr2d"), "solve", os.path.join(REF, "groups", group), str(D), str(Nt),
+                        str(Nx), man, "0", str(len(cells)), solve_args[0], "20", "30", str(sd)], cwd=td)
+            rows = [l.split() for l in sol2.splitlines() if l.startswith("ANE2:")]
+            a1s.append([float(r[3]) for r in rows])
+            a2s.append([float(r[4]) for r in rows])
     rel = [l.replace(cfgdir + "/", "cfg/") for l in cells]
     np.savez_compressed(os.path.join(HERE, f"llr2d_{name}.npz"), group=group, D=D, Nt=Nt, Nx=Nx,
                         seed_args=np.array(seed_args), solve_args=np.array(solve_args), manifest=np.array(rel),
                         ane2=np.array([l for l in sol.splitlines() if l.startswith("ANE2:")]),
-                        head=np.array([l for l in out.splitlines() if l.startswith("LLR2D")]), **cfgs)
+                        head=np.array([l for l in out.splitlines() if l.startswith("LLR2D")]),
+                        stats_a1=np.array(a1s), stats_a2=np.array(a2s), stats_seeds=np.arange(100, 108), **cfgs)
 
 
 def main():

@@ -117,3 +117,100 @@ def test_llr_statistics_match_reference_llr(dsa, dgroups):
     say(f"  a_n: reference {m_ref:.5f} +- {sd_ref:.5f} (n={n_ref}); device {a.mean():.5f} +- {a.std(ddof=1):.5f}; "
         f"z = {z:.2f}")
     assert z < 2.0
+
+
+# axis 3: box (Ewin, hw) x (Ewin2, hw2) on (S1, S2), llr2d.cpp
+CASES2D = [
+    # group, D, Nt, Nx, E1, hw1, E2, hw2, a0, a20, seed, start, K, NRM
+    ("mys1080-v4", 4, 3, 3, -500.0, 40.0, -400.0, 40.0, 1.0, 1.0, 21, "hot", 5, 4),
+    ("mys1080-v4", 4, 4, 4, -1200.0, 90.0, -1300.0, 120.0, 1.6, 0.3, 22, "cold", 4, 3),
+    ("S216", 4, 4, 4, -900.0, 80.0, -700.0, 100.0, 1.0, 0.5, 23, "hot", 4, 3),
+]
+
+
+def params2d(dsa, c):
+    name, D, Nt, Nx, E1, hw1, E2, hw2, a0, a20, seed, start, K, NRM = c
+    return dsa.dsa.llr_params(D, Nt, Nx, 3, 0.0, E1, hw1, a0, seed, start=start, K=K, NRM=NRM, Ewin2=E2, hw2=hw2,
+                              a20=a20)
+
+
+def oracle2d(oracle, ogroups, c, links=None):
+    name, D, Nt, Nx, E1, hw1, E2, hw2, a0, a20, seed, start, K, NRM = c
+    return oracle.OLlr(ogroups(name), D, Nt, Nx, 3, 0.0, E1, hw1, a0, seed, start=start, K=K, NRM=NRM, Ewin2=E2,
+                       hw2=hw2, a20=a20, links=links)
+
+
+@pytest.mark.parametrize("case", CASES2D, ids=lambda c: f"{c[0]}-{c[2]}x{c[3]}-{c[11]}")
+def test_llr2d_matches_oracle(dsa, dgroups, oracle, ogroups, case):
+    say("case llr2d vs oracle", case)
+    o = oracle2d(oracle, ogroups, case)
+    with dsa.dsa.LlrEngine(4, 2) as eng:
+        s = eng.add(dgroups(case[0]), params2d(dsa, case))
+        for sweep in range(30):
+            eng.step(1)
+            o.step(1)
+            same(eng.status(s), o.status())
+            assert eng.status(s)["a2"] == o.status()["a2"]
+            assert np.array_equal(eng.links(s), o.a_links), f"links differ after sweep {sweep + 1}"
+
+
+def test_llr2d_from_seeded_cells_matches_oracle(dsa, dgroups, oracle, ogroups):
+    """Starting from llr2d's own seeded cell configurations (the solve path)."""
+    f = np.load(os.path.join(GOLDEN, "llr2d_seed_3x3.npz"))
+    from dym_symanzik_action_amd import cfgio
+
+    cells = cfgio.parse_manifest([str(l) for l in f["manifest"]])[:3]
+    say("case llr2d from seeded cells", len(cells))
+    with dsa.dsa.LlrEngine(4, 4) as eng:
+        slots, orc = [], []
+        for k, c in enumerate(cells):
+            links = f[os.path.basename(c["path"])[:-4]]
+            case = ("mys1080-v4", 4, 3, 3, c["E1"], c["hw1"], c["E2"], c["hw2"], 1.0, 1.0, 300 + k, "hot", 5, 3)
+            slots.append(eng.add(dgroups("mys1080-v4"), params2d(dsa, case), links=links))
+            orc.append(oracle2d(oracle, ogroups, case, links=links))
+        for _ in range(4):
+            eng.step(5)
+            for s, o in zip(slots, orc):
+                o.step(5)
+                same(eng.status(s), o.status())
+                assert eng.status(s)["a2"] == o.status()["a2"]
+                assert np.array_equal(eng.links(s), o.a_links)
+
+
+def test_llr2d_statistics_match_reference_llr2d(dsa, dgroups):
+    """llr2d's seed cells on 3^4 (tests/golden/llr2d_seed_3x3.npz), each solved
+    with K = 20, NRM = 30 from a0 = a20 = 1.0: the reference's a_n over eight
+    seeds and 16 device chains per cell agree within 4 standard errors for
+    both tilts."""
+    f = np.load(os.path.join(GOLDEN, "llr2d_seed_3x3.npz"))
+    from dym_symanzik_action_amd import cfgio
+
+    cells = cfgio.parse_manifest([str(l) for l in f["manifest"]])
+    ref1, ref2 = f["stats_a1"], f["stats_a2"]  # [seed, cell]
+    nrep = 16
+    say("case llr2d statistics", len(cells), "cells x", nrep)
+    with dsa.dsa.LlrEngine(4, len(cells) * nrep) as eng:
+        slots = []
+        for k, c in enumerate(cells):
+            links = f[os.path.basename(c["path"])[:-4]]
+            for r in range(nrep):
+                case = ("mys1080-v4", 4, 3, 3, c["E1"], c["hw1"], c["E2"], c["hw2"], 1.0, 1.0, 5000 + 100 * k + r,
+                        "hot", 20, 30)
+                slots.append(eng.add(dgroups("mys1080-v4"), params2d(dsa, case), links=links))
+        for _ in range(400):
+            eng.step(5)
+            if all(eng.status(s)["phase"] == 2 for s in slots):
+                break
+        st = [eng.status(s) for s in slots]
+    assert all(s["phase"] == 2 and s["reached"] == 1 and s["iterations"] == 30 for s in st)
+    a1 = np.array([s["a"] for s in st]).reshape(len(cells), nrep)
+    a2 = np.array([s["a2"] for s in st]).reshape(len(cells), nrep)
+    worst = 0.0
+    for k in range(len(cells)):
+        for mine, ref in ((a1[k], ref1[:, k]), (a2[k], ref2[:, k])):
+            se = np.hypot(mine.std(ddof=1) / np.sqrt(mine.size), ref.std(ddof=1) / np.sqrt(ref.size))
+            z = abs(mine.mean() - ref.mean()) / se
+            say(f"  cell {k}: device {mine.mean():.5f}+-{mine.std(ddof=1):.5f}  "
+                f"reference {ref.mean():.5f}+-{ref.std(ddof=1):.5f}  z={z:.2f}")
+            worst = max(worst, z)
+    assert worst < 4.0
