@@ -283,7 +283,7 @@ __global__ void k_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n
   if (S.phase == kLlrSeed) {
     S.seed_sweeps += 1;
     if (llr_inside(LC, S.S1, S.S2)) {
-      S.phase = kLlrRM;
+      S.phase = LC.NRM ? kLlrRM : kLlrDone;  // NRM = 0: drive-in only (llr2d.cpp:192-202)
       S.reached = 1;
       S.k = 0;
       S.m = 0;

@@ -210,7 +210,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   dsa_chain_params cp{p->D, p->Nt, p->Nx, 0, 0, 0, p->seed, p->start, DSA_RNG_PHILOX, DSA_ORDER_COLOUR};
   if (int rc = dsa::check_params(&cp)) return rc;
   if (p->axis != 1 && p->axis != 2 && p->axis != 3) return fail(DSA_E_ARG, "axis must be 1, 2 or 3");
-  if (!(p->hw > 0) || !p->K || !p->NRM) return fail(DSA_E_ARG, "need hw > 0, K > 0, NRM > 0");
+  if (!(p->hw > 0) || !p->K) return fail(DSA_E_ARG, "need hw > 0 and K > 0");
   if (p->axis == 3 && !(p->hw2 > 0)) return fail(DSA_E_ARG, "axis 3 needs hw2 > 0");
   const uint64_t bytes = footprint(*p);
   if (L->budget && L->used_bytes + bytes > L->budget) return fail(DSA_E_RANGE, "HBM budget exhausted");

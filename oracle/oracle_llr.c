@@ -445,7 +445,7 @@ static void dev_sweep_end(ogl_dev *L) {
   if (L->phase == OGL_SEED) {
     L->seed_sweeps++;
     if (dev_inside(L, L->S1, L->S2)) {
-      L->phase = OGL_RM;
+      L->phase = L->NRM ? OGL_RM : OGL_DONE; /* NRM = 0: drive-in only */
       L->reached = 1;
       L->k = 0;
       L->m = 0;

@@ -144,3 +144,23 @@ def test_cli_usage_and_errors():
     r = subprocess.run([CLI, group_file("Q8"), "4", "2", "2", "0", "1", "0", "1", "--rng", "bogus"],
                        capture_output=True, text=True)
     assert r.returncode == 1
+
+
+def test_llr_cli_usage_and_errors(tmp_path):
+    """llr / llr2d (one binary, mode by name and first argument): short
+    argument lists print the reference's usage on stderr and exit 1
+    (llr.cpp:278-283, llr2d.cpp:297-304, 420-423, 478-481); a solve pass
+    without its manifest exits 1 (llr2d.cpp:490)."""
+    llr = os.path.join(REPO, "dym_symanzik_action_amd", "build", "llr")
+    llr2d = os.path.join(REPO, "dym_symanzik_action_amd", "build", "llr2d")
+    r = subprocess.run([llr, "g", "4"], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stderr.startswith(f"usage: {llr} group D Nt Nx axis bg") and r.stdout == ""
+    r = subprocess.run([llr2d, "g"], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stderr.startswith(f"usage: {llr2d} group D Nt Nx E1top E1bot")
+    r = subprocess.run([llr2d, "seed", "g"], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stderr.startswith("usage: seed seed group D Nt Nx")
+    r = subprocess.run([llr2d, "solve", "g"], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stderr.startswith("usage: solve solve group D Nt Nx manifest lo hi a0 K NRM seed")
+    r = subprocess.run([llr2d, "solve", group_file("Q8"), "4", "3", "3", str(tmp_path / "none.txt"), "0", "1", "1.0",
+                        "2", "2", "1"], capture_output=True, text=True)
+    assert r.returncode == 1 and "solve: cannot open manifest" in r.stderr
