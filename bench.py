@@ -11,7 +11,9 @@ Prints one JSON line.  Besides the contract fields it carries
   roofline      the sweep kernel against HBM (66 algorithmic bytes of link
                 traffic per link-update), with per-launch times from HIP
                 events on the chain's stream, and its random-gather ceiling
-                (608 table gathers per link-update vs a measured probe);
+                (table gathers per link-update vs a measured probe: 608 for
+                the table kernel; 108 + 25 per accepted hit + 98 per exact
+                re-decision for the class kernel);
   cpu_baseline  the reference binary (oracle/_ref/dym-mod-metro, built from
                 the reference's sources) timed on this host's cores for two
                 sweeps of the same workload, by its own TM_END: update line.
@@ -36,19 +38,26 @@ BETA = (0.0, 3.0, -0.25)
 SEED = 11
 NHIT = 20
 BYTES_PER_LINK_UPDATE = 66      # 64 B of link reads + 2 B write per updated link
-GATHERS_PER_LINK_UPDATE = 608   # 84 staple + 24 initial + 20*25 hit-loop table gathers
+GATHERS_TABLE = 608     # 84 staple + 24 initial + 20*25 hit-loop table gathers
+GATHERS_CLASS = 108     # 84 staple + 24 staple products with the link; per hit: LDS only
+GATHERS_CLASS_ACCEPT = 25   # new link + 24 staple products moved by the accepted factor
+GATHERS_CLASS_EXACT = 98    # b', b^-1 and 4 per staple for an exact re-decision
+KERNEL_NAME = {"table": "k_sweep<4>", "class": "k_sweep_cls<4, true>", "class-narrow": "k_sweep_cls<4, false>"}
 HBM_PEAK_GBS = 8000.0
-PMC_FILE = os.path.join(REPO, "profiles", "k_sweep_32x64_pmc.txt")
+PMC_FILE = {"table": "k_sweep_32x64_pmc.txt", "class-narrow": "k_sweep_cls_narrow_32x64_pmc.txt"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary."""
+def read_traffic(form):
+    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary
+    of the kernel form that ran (None where none is committed)."""
+    if form not in PMC_FILE:
+        return None
     try:
-        for line in open(PMC_FILE):
+        for line in open(os.path.join(REPO, "profiles", PMC_FILE[form])):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -221,12 +230,14 @@ def main():
     log("bench: creating the 32^3x64 chain")
     c = dsa.Chain(g, D, NT, NX, *BETA, SEED, start="hot", rng="philox", order="colour")
     c.set_kernel(args.kernel)
+    form = c.kernel_form()
     npass, ncol = c.schedule_info()
     log(f"bench: {npass} passes per sweep, colours per direction {ncol}; warmup {args.warmup}")
     import ctypes as C
 
     c.update(args.warmup)
     hit0, acc0 = c.counters()
+    ex0 = c.exact_decisions()
     log(f"bench: timing {args.steps} sweeps")
     # every pass launch of the timed sweeps is bracketed by HIP events on the
     # chain's stream: the roofline's per-launch time comes from the same sweeps
@@ -239,13 +250,19 @@ def main():
     dsa.lib().dsa_chain_kernel_time(c._h, C.byref(ms), C.byref(nl))
     dsa.lib().dsa_chain_set_profiling(c._h, 0)
     hit1, acc1 = c.counters()
+    ex1 = c.exact_decisions()
     assert hit1 - hit0 == args.steps * LINKS * NHIT
     value = args.steps * LINKS / elapsed
     avg_launch_s = ms.value * 1e-3 / nl.value
     units = LINKS // npass
     achieved = BYTES_PER_LINK_UPDATE * units / avg_launch_s / 1e9
-    traffic = read_traffic()
-    gather_rate = GATHERS_PER_LINK_UPDATE * units / avg_launch_s
+    traffic = read_traffic(form)
+    if form == "table":
+        gathers_per_unit = float(GATHERS_TABLE)
+    else:
+        gathers_per_unit = GATHERS_CLASS + (GATHERS_CLASS_ACCEPT * (acc1 - acc0) +
+                                            GATHERS_CLASS_EXACT * (ex1 - ex0)) / (args.steps * LINKS)
+    gather_rate = gathers_per_unit * units / avg_launch_s
     gp = C.c_double()
     best = 0.0
     for blocks in (2048, 4096, 8192):
@@ -299,11 +316,16 @@ def main():
         "plaquette_after": m[2],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sweep<4>", "avg_launch_ms": avg_launch_s * 1e3, "units_per_launch": units,
+                     "kernel": KERNEL_NAME[form], "avg_launch_ms": avg_launch_s * 1e3, "units_per_launch": units,
                      "algorithmic_bytes_per_unit": BYTES_PER_LINK_UPDATE,
                      "gather": {"achieved_per_s": gather_rate, "ceiling_per_s": best,
                                 "frac": gather_rate / best if best else None,
-                                "per_unit": GATHERS_PER_LINK_UPDATE}},
+                                "per_unit": gathers_per_unit,
+                                # the reference's arithmetic needs 608 per link-update:
+                                # the rate it would have to sustain for this throughput
+                                "reference_equivalent_per_s": GATHERS_TABLE * value,
+                                "reference_equivalent_frac": GATHERS_TABLE * value / best if best else None},
+                     "exact_redecisions_per_hit": (ex1 - ex0) / (hit1 - hit0)},
         "cpu_baseline": cpu,
         "batched_vs_serial": scan,
         "llr_serving": serve,

@@ -184,6 +184,19 @@ int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
 
 namespace {
 
+// The class form a chain's large passes run (kClsNone: the table kernel).
+// CLASS takes the int16-value form where it fits, else the byte-index one;
+// AUTO prefers the byte-index form (half the LDS of the int16 one), which
+// both certifies more decisions and measures faster on S(1080).
+int cls_mode(const dsa_chain *c) {
+  switch (c->kernel) {
+    case DSA_KERNEL_CLASS: return c->cls_best;
+    case DSA_KERNEL_CLASS_NARROW: return dsa::kClsNarrow;
+    case DSA_KERNEL_AUTO: return c->narrow_ok && c->margin8 < 1e-3 ? (int)dsa::kClsNarrow : c->cls_best;
+    default: return dsa::kClsNone;
+  }
+}
+
 void free_tables(DeviceTables &t) {
   if (t.mult) (void)hipFree(t.mult);
   if (t.inv) (void)hipFree(t.inv);
@@ -403,10 +416,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.seed = (uint32_t)c->p.seed;
   A.fall = c->d_fall;
   // the class form this chain runs for passes large enough to fill the device
-  int mode = dsa::kClsNone;
-  if (c->kernel == DSA_KERNEL_CLASS) mode = c->cls_best;
-  if (c->kernel == DSA_KERNEL_CLASS_NARROW) mode = dsa::kClsNarrow;
-  if (c->kernel == DSA_KERNEL_AUTO) mode = c->cls_best;
+  const int mode = cls_mode(c);
   A.cmargin = (mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
   const size_t np = c->S.passes.size();
   if (c->profiling && c->pev.size() < 2 * np) {
@@ -448,6 +458,13 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   if (n) c->lines_valid = false;
+  return 0;
+}
+
+int dsa_chain_kernel_form(dsa_chain *c, int32_t *form) {
+  if (!c || !form) return fail(DSA_E_ARG, "dsa_chain_kernel_form: NULL argument");
+  const int mode = cls_mode(c);
+  *form = mode == dsa::kClsWide ? DSA_KERNEL_CLASS : mode == dsa::kClsNarrow ? DSA_KERNEL_CLASS_NARROW : DSA_KERNEL_TABLE;
   return 0;
 }
 

@@ -104,6 +104,7 @@ _SIGS = {
     "dsa_chain_load_cfg": (C.c_int, [C.c_void_p, C.c_char_p]),
     "dsa_chain_save_kentucky": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_uint64]),
     "dsa_chain_exact_decisions": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "dsa_chain_kernel_form": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "dsa_chain_kernel_time": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "dsa_probe_gather": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
     "dsa_probe_copy": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
@@ -264,6 +265,13 @@ class Chain:
         byte class-index form), or "auto"; all give identical links.  A huge
         margin_scale re-decides every hit exactly."""
         _check(lib().dsa_chain_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
+
+    def kernel_form(self) -> str:
+        """The kernel passes of at least 2048 links run: "table", "class" (the
+        int16-value form) or "class-narrow"."""
+        f = C.c_int32()
+        _check(lib().dsa_chain_kernel_form(self._h, C.byref(f)))
+        return {v: k for k, v in DSA_KERNEL.items()}[f.value]
 
     def exact_decisions(self) -> int:
         n = C.c_uint64()

@@ -147,7 +147,9 @@ def test_config2_shape_one_sweep_matches_oracle(dsa, dgroups, oracle, ogroups):
     o = oracle.OChain(ogroups("mys1080-v4"), 4, 32, 16, 0.0, 3.0, -0.25, 7, rng="philox", order="colour")
     with dsa.Chain(dgroups("mys1080-v4"), 4, 32, 16, 0.0, 3.0, -0.25, 7) as c:
         assert np.array_equal(c.a, o.a)
-        c.update(1)  # the default kernel choice takes the class kernel at this size
+        # the default choice takes the byte-index class kernel at this size
+        assert c.kernel_form() == "class-narrow"
+        c.update(1)
         o.sweep(1)
         assert np.array_equal(c.a, o.a)
         assert c.counters() == o.counters()
