@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,8 @@ struct HostChain {
   dsa::Schedule S;
   dsa::DeviceTables dt;
   size_t links_off = 0, sites_off = 0, ms_off = 0;
+  bool cls = false;     // class-product update
+  double margin = 0;    // its decision bound at margin scale 1
 };
 
 }  // namespace
@@ -56,6 +59,9 @@ struct dsa_batch {
   std::vector<uint32_t> pass_off, pass_n;  // tasks of pass q: [pass_off[q], pass_off[q] + pass_n[q])
   uint32_t nobs = 0;
   size_t lds = 0;
+  int kernel = DSA_KERNEL_AUTO;
+  double margin_scale = 1.0;
+  unsigned long long *d_fall = nullptr;
   uint64_t sweeps = 0;
   double last_ms = 0;
   hipGraphExec_t graph = nullptr;
@@ -113,13 +119,23 @@ int dsa_batch_finalize(dsa_batch *b) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DSA_E_DEVICE, "no HIP device visible");
   HIP_TRY(hipGetDevice(&b->device));
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device) != hipSuccess) lds_max = 0;
   size_t nlinks = 0, nsites = 0, nms = 0, maxlinks = 0;
   size_t maxpass = 0;
   for (HostChain &hc : b->chains) {
     if (int rc = dsa::upload_tables(hc.grp, b->device, hc.dt)) return rc;
     const dsa::Tables T = dsa::tables_view(hc.grp, hc.dt);
     if (dsa::sweep_lds_bytes(T) > 65536) return fail(DSA_E_RANGE, "group order too large for the LDS tables");
-    b->lds = std::max(b->lds, dsa::sweep_lds_bytes(T));
+    // the byte-index class update where its tables fit beside the static class
+    // values and its decision bound is small, as a lone chain's AUTO choice
+    hc.margin = T.cprod ? dsa::class_margin(hc.grp->g, (int)hc.p.D, hc.p.beta0, hc.p.beta1, hc.p.beta2, 0x1p-25) : 0;
+    hc.cls = b->kernel != DSA_KERNEL_TABLE && T.cprod &&
+             dsa::sweep_cls_lds_bytes(T, dsa::kClsNarrow) + 256 <= (size_t)lds_max &&
+             (b->kernel != DSA_KERNEL_AUTO || hc.margin < 1e-3);
+    if (b->kernel != DSA_KERNEL_TABLE && b->kernel != DSA_KERNEL_AUTO && !hc.cls)
+      return fail(DSA_E_STATE, "no class tables for a chain's group on this device");
+    b->lds = std::max(b->lds, hc.cls ? dsa::sweep_cls_lds_bytes(T, dsa::kClsNarrow) : dsa::sweep_lds_bytes(T));
     std::string err = hc.p.order == DSA_ORDER_COLOUR ? dsa::build_colour_schedule(hc.G, hc.S)
                                                      : dsa::build_reference_schedule(hc.G, hc.S);
     if (!err.empty()) return fail(DSA_E_STATE, err);
@@ -143,6 +159,8 @@ int dsa_batch_finalize(dsa_batch *b) {
   const size_t nc = b->chains.size();
   HIP_TRY(hipMalloc(&b->d_acc, nc * sizeof(unsigned long long)));
   HIP_TRY(hipMalloc(&b->d_sweep, sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&b->d_fall, sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(b->d_fall, 0, sizeof(unsigned long long)));
   HIP_TRY(hipMemset(b->d_acc, 0, nc * sizeof(unsigned long long)));
   HIP_TRY(hipMemset(b->d_sweep, 0, sizeof(unsigned long long)));
 
@@ -163,6 +181,9 @@ int dsa_batch_finalize(dsa_batch *b) {
     d.seed = (uint32_t)hc.p.seed;
     d.rng = hc.p.rng == DSA_RNG_MINSTD ? dsa::kRngMinstd : dsa::kRngPhilox;
     d.sweep_base = 0;
+    d.cls = hc.cls ? 1 : 0;
+    d.cmargin = hc.margin * b->margin_scale;
+    d.fall = b->d_fall;
     HIP_TRY(hipMemcpy(b->d_sites + hc.sites_off, hc.S.sites.data(), hc.S.sites.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
     // start: the same kernels and host draws as a lone chain
@@ -225,12 +246,37 @@ void dsa_batch_destroy(dsa_batch *b) {
   if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
   for (void *p : {(void *)b->d_links, (void *)b->d_sites, (void *)b->d_mstate, (void *)b->d_chains,
                   (void *)b->d_tasks, (void *)b->d_obs, (void *)b->d_obs_first, (void *)b->d_obs_n,
-                  (void *)b->d_acc, (void *)b->d_sweep, (void *)b->d_partials, (void *)b->d_out, (void *)b->d_aos})
+                  (void *)b->d_acc, (void *)b->d_sweep, (void *)b->d_partials, (void *)b->d_out, (void *)b->d_aos,
+                  (void *)b->d_fall})
     if (p) (void)hipFree(p);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
+}
+
+int dsa_batch_set_kernel(dsa_batch *b, int32_t kernel, double margin_scale) {
+  if (!b) return fail(DSA_E_ARG, "dsa_batch_set_kernel: NULL batch");
+  if (b->final) return fail(DSA_E_STATE, "dsa_batch_set_kernel after dsa_batch_finalize");
+  if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS &&
+      kernel != DSA_KERNEL_CLASS_NARROW)
+    return fail(DSA_E_ARG, "dsa_batch_set_kernel: unknown kernel " + std::to_string(kernel));
+  if (!(margin_scale >= 1.0) || !std::isfinite(margin_scale))
+    return fail(DSA_E_ARG, "dsa_batch_set_kernel: margin_scale must be finite and >= 1");
+  b->kernel = kernel;
+  b->margin_scale = margin_scale;
+  return 0;
+}
+
+int dsa_batch_exact_decisions(dsa_batch *b, uint64_t *n) {
+  if (!b || !n) return fail(DSA_E_ARG, "dsa_batch_exact_decisions: NULL argument");
+  if (!b->final) return fail(DSA_E_STATE, "batch not finalized");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  unsigned long long v = 0;
+  HIP_TRY(hipMemcpy(&v, b->d_fall, sizeof v, hipMemcpyDeviceToHost));
+  *n = v;
+  return 0;
 }
 
 int dsa_batch_sweep(dsa_batch *b, uint32_t n) {

@@ -89,15 +89,26 @@ dsa::Tables dsa::tables_view(const dsa_group *g, const DeviceTables &t) {
   return T;
 }
 
-double dsa::class_margin(const Group &g, int D, double beta0, double beta1, double beta2, double q) {
+void dsa::class_margin_terms(const Group &g, int D, double q, double &e1, double &e2, double &f1, double &f2,
+                             double &r1, double &r2) {
   const double u = 0x1p-53;
   const int ns = 2 * (D - 1), nr = 6 * (D - 1);
   auto gam = [u](int n) { return n * u / (1.0 - n * u); };
   // |r1_exact - r1_class| <= e1 + f1: value differences (exact bounds) plus
   // the rounding of the reference's float sum; likewise m1
-  const double e1 = ns * (g.dmax + q), f1 = gam(ns) * ns * g.rmax;
-  const double e2 = nr * (g.dmax + q), f2 = gam(nr) * nr * g.rmax;
-  const double a = std::fabs(beta0) + std::fabs(beta1) * (ns * g.rmax + e1) + std::fabs(beta2) * (nr * g.rmax + e2);
+  e1 = ns * (g.dmax + q);
+  f1 = gam(ns) * ns * g.rmax;
+  e2 = nr * (g.dmax + q);
+  f2 = gam(nr) * nr * g.rmax;
+  r1 = ns * g.rmax + e1;
+  r2 = nr * g.rmax + e2;
+}
+
+double dsa::class_margin(const Group &g, int D, double beta0, double beta1, double beta2, double q) {
+  const double u = 0x1p-53;
+  double e1, e2, f1, f2, r1, r2;
+  class_margin_terms(g, D, q, e1, e2, f1, f2, r1, r2);
+  const double a = std::fabs(beta0) + std::fabs(beta1) * r1 + std::fabs(beta2) * r2;
   const double vals = std::fabs(beta1) * e1 + std::fabs(beta2) * e2;                   // per action
   const double rounding = std::fabs(beta1) * f1 + std::fabs(beta2) * f2 + 8.0 * u * a;  // per action
   // both actions of a hit; the rounding terms with a safety factor of 4

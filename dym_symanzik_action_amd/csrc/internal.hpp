@@ -42,6 +42,10 @@ Tables tables_view(const dsa_group *g, const DeviceTables &t);
 // and of beta0 + beta1 r1 + beta2 m1, with a safety factor of 4.
 // q: the quantisation of the class values (half their unit).
 double class_margin(const Group &g, int D, double beta0, double beta1, double beta2, double q);
+// The terms of class_margin per coupling: value spread and quantisation,
+// sum rounding and magnitude of the plaquette (1) and rectangle (2) sums.
+void class_margin_terms(const Group &g, int D, double q, double &e1, double &e2, double &f1, double &f2, double &r1,
+                        double &r2);
 // Shape and option checks shared by chains and batches.
 int check_params(const dsa_chain_params *p);
 // minstd-compat start (dym-mod-metro.cpp:227-228, 253-255): per-site states and

@@ -111,21 +111,47 @@ __global__ __launch_bounds__(kBlock) void k_sweep_batch(const BatchChain *chains
                                                         const unsigned long long *sweep_ctr,
                                                         unsigned long long *acc_out) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_rcq[64];
   const BatchTask tk = tasks[blockIdx.x];
   const BatchChain &BC = chains[tk.chain];
-  const uint32_t P = BC.T.P;
-  double *s_retr = reinterpret_cast<double *>(smem);
-  uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_retr + P);
-  uint16_t *s_pool = s_inv + P;
-  load_tables_lds(BC.T, s_retr, s_inv, s_pool);
-  __syncthreads();
+  const uint32_t P = BC.T.P, np = BC.T.npool;
   const uint64_t sweep = BC.sweep_base + *sweep_ctr;
-  const TabView T{BC.T.mult, s_inv, P};
-  const LinkCtx C{BC.links, BC.mstate, s_retr, s_pool, (double)BC.T.npool, BC.beta0, BC.beta1, BC.beta2,
-                  BC.seed, (uint32_t)sweep, (uint32_t)(sweep >> 32), BC.rng};
   unsigned long long acc = 0;
-  for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
-    acc += update_link<D>(C, BC.G, T, tk.mu, BC.sites[tk.offset + t]);
+  if (BC.cls) {  // the layout of k_sweep_cls's narrow form (uniform over the block)
+    uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *s_pool = reinterpret_cast<uint16_t *>(smem + pad16(2 * (size_t)P));
+    unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
+    unsigned char *s_tab = s_tab0 + pad16(P);
+    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+      s_inv[t] = BC.T.inv[t];
+      s_tab0[t] = BC.T.cls[t];
+    }
+    for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) s_pool[t] = BC.T.pool[t];
+    for (uint32_t t = threadIdx.x; t < BC.T.ncls; t += blockDim.x) s_rcq[t] = BC.T.rcq[t];
+    copy16(s_tab, BC.T.cprod, (size_t)np * P);
+    __syncthreads();
+    const TabView T{BC.T.mult, s_inv, P};
+    const LinkCtx C{BC.links, BC.mstate, nullptr, s_pool, (double)np, BC.beta0, BC.beta1, BC.beta2,
+                    BC.seed, (uint32_t)sweep, (uint32_t)(sweep >> 32), BC.rng};
+    const ClsCtx K{BC.T.retr, BC.cmargin};
+    const ClsNarrow Lk{s_tab, s_tab0, s_rcq};
+    uint32_t nfall = 0;
+    for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
+      acc += update_link_cls<D>(C, Lk, K, BC.G, T, tk.mu, BC.sites[tk.offset + t], nfall);
+    const unsigned long long nf = wave_sum_u64(nfall);
+    if ((threadIdx.x & 63u) == 0 && nf) atomicAdd(BC.fall, nf);
+  } else {
+    double *s_retr = reinterpret_cast<double *>(smem);
+    uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_retr + P);
+    uint16_t *s_pool = s_inv + P;
+    load_tables_lds(BC.T, s_retr, s_inv, s_pool);
+    __syncthreads();
+    const TabView T{BC.T.mult, s_inv, P};
+    const LinkCtx C{BC.links, BC.mstate, s_retr, s_pool, (double)np, BC.beta0, BC.beta1, BC.beta2,
+                    BC.seed, (uint32_t)sweep, (uint32_t)(sweep >> 32), BC.rng};
+    for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
+      acc += update_link<D>(C, BC.G, T, tk.mu, BC.sites[tk.offset + t]);
+  }
   acc = wave_sum_u64(acc);
   if ((threadIdx.x & 63u) == 0 && acc) atomicAdd(&acc_out[tk.chain], acc);
 }

@@ -76,6 +76,11 @@ struct BatchChain {
   uint32_t seed;
   int rng;
   uint64_t sweep_base;    // sweeps done before joining the batch counter
+  // class-product update (byte tables in LDS): on or off, its decision
+  // bound, and the batch's count of hits re-decided from element values
+  int cls;
+  double cmargin;
+  unsigned long long *fall;
 };
 
 // A block's work: `count` (<= 256) consecutive sites of one chain's pass.
@@ -120,6 +125,12 @@ struct LlrChain {
   double tol;        // stop once |delta a| < tol (0: run NRM iterations)
   uint32_t cap;      // drive-in sweep cap
   uint32_t npass;    // passes per sweep of this chain
+  // class path (byte class tables in LDS, certified decisions): on or off,
+  // and the terms of the bound on |element - class| sums: value spread and
+  // quantisation (ce), element-sum rounding (cf), magnitude (cr) of the
+  // plaquette (1) and rectangle (2) parts; cmscale multiplies the bound
+  int cls;
+  double cmscale, ce1, ce2, cf1, cf2, cr1, cr2;
 };
 
 struct LlrState {
@@ -134,6 +145,13 @@ struct LlrState {
   uint64_t sweep;    // Philox sweep counter
   unsigned long long acc, hit;
 };
+
+// LDS of the LLR pass kernel for one group: ReTr, inverses and pool, plus
+// (class path) the class of each element and the npool x P class table.
+__host__ __device__ inline size_t llr_lds_bytes(const Tables &T, bool cls) {
+  const size_t e = (size_t)T.P * sizeof(double) + (size_t)T.P * sizeof(uint16_t) + (size_t)T.npool * sizeof(uint16_t);
+  return cls ? pad16(e) + pad16(T.P) + pad16((size_t)T.npool * T.P) : e;
+}
 
 // Proposals: every link of pass `pass` of every active chain.
 hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const BatchTask *tasks, uint32_t ntasks,

@@ -91,23 +91,60 @@ __device__ inline LlrRule llr_rule_of(const LlrChain &LC, const LlrState &S) {
   return R;
 }
 
+// Bound on |exact - class| for A1 dS1 + A2 dS2 computed from class sums
+// instead of the element-value sums of update (the within-class spread of the
+// table's ReTr values, the quantisation of the class values, and the rounding
+// of the element sums and of the combination, the latter with a factor 4).
+__device__ inline double llr_margin(const LlrChain &LC, double A1, double A2) {
+  A1 = fabs(A1);
+  A2 = fabs(A2);
+  const double vals = A1 * LC.ce1 + A2 * LC.ce2;
+  const double rounding = A1 * LC.cf1 + A2 * LC.cf2 + 8.0 * 0x1p-53 * (A1 * LC.cr1 + A2 * LC.cr2);
+  return LC.cmscale * (2.0 * vals * (1.0 + 1e-6) + 8.0 * rounding);
+}
+
+// Certified comparison exp(-x) > u from x_c with |x - x_c| <= M (dsa_exp within
+// one ulp on both sides): 1 true, 0 false, -1 undecided.
+__device__ inline int exp_gt_certified(double xc, double M, double u) {
+  const double e = dsa_exp(-xc), w = 1.02 * M + 0x1p-46;
+  if (w >= 0.5) return -1;
+  return u < e * (1.0 - w) ? 1 : (u >= e * (1.0 + w) ? 0 : -1);
+}
+
+// Certified sign test y > 0 from y_c with |y - y_c| <= M.
+__device__ inline int gt0_certified(double yc, double M) { return yc - M > 0.0 ? 1 : (yc + M <= 0.0 ? 0 : -1); }
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, LlrState *st, const BatchTask *tasks) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_rcq[64];  // class values (2^-24), indexed by the byte tables
   const BatchTask tk = tasks[blockIdx.x];
   const LlrChain &LC = chains[tk.chain];
   const LlrState &S = st[tk.chain];
   if (!llr_active(S.phase)) return;  // uniform over the block
   const BatchChain &BC = LC.base;
-  const uint32_t P = BC.T.P;
+  const uint32_t P = BC.T.P, np = BC.T.npool;
   double *s_retr = reinterpret_cast<double *>(smem);
   uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_retr + P);
   uint16_t *s_pool = s_inv + P;
   load_tables_lds(BC.T, s_retr, s_inv, s_pool);
+  // class path: class of each element and of pool[j] * t, after the element tables
+  const bool cls = LC.cls != 0;
+  uint8_t *s_cls = smem + pad16(llr_lds_bytes(BC.T, false));
+  uint8_t *s_cprod = s_cls + pad16(P);
+  if (cls) {
+    for (uint32_t t = threadIdx.x; t < BC.T.ncls; t += blockDim.x) s_rcq[t] = BC.T.rcq[t];
+    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) s_cls[t] = BC.T.cls[t];
+    const uint32_t n16 = (uint32_t)(pad16((size_t)np * P) / 16);
+    const uint4 *src = reinterpret_cast<const uint4 *>(BC.T.cprod);
+    uint4 *dst = reinterpret_cast<uint4 *>(s_cprod);
+    for (uint32_t t = threadIdx.x; t < n16; t += blockDim.x) dst[t] = src[t];
+  }
   __syncthreads();
 
-  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1);
+  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1), NL = NS + NR;
   const TabView T{BC.T.mult, s_inv, P};
+  const ClsNarrow Lk{s_cprod, s_cls, s_rcq};
   const Geom &G = BC.G;
   const int axis = LC.axis;
   const LlrRule R = llr_rule_of(LC, S);
@@ -117,8 +154,19 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
   const double sign = from_fix(axis == 1 ? S.S1 : S.S2) < LC.Ewin ? 1.0 : -1.0;
   const double gx = from_fix(S.S1 - R.mid) / R.hw / R.hw, gy = from_fix(S.S2 - R.mid2) / R.hw2 / R.hw2;
   const double a = S.a, a2 = S.a2, bg = LC.bg;
+  // the class path's test: coefficients of (dS1, dS2) and their bound
+  double A1, A2;
+  if (drive) {
+    A1 = axis == 3 ? gx : axis == 1 ? sign : 0.0;
+    A2 = axis == 3 ? gy : axis == 2 ? sign : 0.0;
+  } else {
+    A1 = axis == 2 ? bg : a;
+    A2 = axis == 3 ? a2 : axis == 1 ? bg : a;
+  }
+  const double margin = cls ? llr_margin(LC, A1, A2) : 0.0;
+  const double A1q = A1 * ClsNarrow::kScale, A2q = A2 * ClsNarrow::kScale;
   const uint32_t sw_lo = (uint32_t)S.sweep, sw_hi = (uint32_t)(S.sweep >> 32);
-  const double npool_d = (double)BC.T.npool;
+  const double npool_d = (double)np;
   for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x) {
     const uint32_t idx = tk.offset + t;
     const uint32_t i = BC.sites[idx];
@@ -129,6 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
     uint32_t b = BC.links[lp];
     const uint32_t b0 = b;
+    // element sums (reference order) in double and in fixed point
     double r1 = 0, m1 = 0;
     long long q1 = 0, q2 = 0;
 #pragma unroll
@@ -145,35 +194,97 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     }
     const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
     uint32_t nacc = 0;
-    for (int h = 0; h < kNHit; ++h) {
-      const U4 w = philox4x32_10(U4{link, (uint32_t)h, sw_lo, sw_hi}, BC.seed, kKeyLLR);
-      const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
-      const uint32_t bnew = T.M(b, s_pool[(uint32_t)(npool_d * u1)]);
-      double r1n = 0, m1n = 0;
+    if (cls) {
+      // t_k = s_k b: ReTr(b p s_k) = ReTr(t_k p), of the class of pool[j] t_k
+      uint32_t tt[NL];
 #pragma unroll
-      for (int j = 0; j < NS; ++j) r1n += s_retr[T.M(bnew, stp[j])];
+      for (int k = 0; k < NL; ++k) tt[k] = T.M(k < NS ? stp[k] : re[k - NS], b);
+      auto sums = [&](ClsNarrow::Row row, int32_t &rc, int32_t &mc) {
+        int32_t x = 0, y = 0;
 #pragma unroll
-      for (int q = 0; q < NR; ++q) m1n += s_retr[T.M(bnew, re[q])];
-      const double dS1 = r1n - r1, dS2 = m1n - m1;
-      bool accept;
-      if (axis == 3) {
-        if (drive)
-          accept = gx * dS1 + gy * dS2 < 0.0;  // greedy move toward the cell centre
-        else
-          accept = dsa_exp(-a * dS1 - a2 * dS2) > u2;  // llr2d.cpp:211-212
-      } else {
-        const double dEw = axis == 1 ? dS1 : dS2;
-        const double dEbg = axis == 1 ? dS2 : dS1;
-        if (drive)
-          accept = sign * dEw > 0.0;  // greedy move toward the window (llr.cpp:208-209)
-        else
-          accept = dsa_exp(-a * dEw - bg * dEbg) > u2;  // llr.cpp:211-212
+        for (int k = 0; k < NL; ++k) {
+          const int32_t v = Lk.val(row, tt[k]);
+          if (k < NS) x += v; else y += v;
+        }
+        rc = x;
+        mc = y;
+      };
+      int32_t r1c, m1c;
+      sums(Lk.row0(), r1c, m1c);
+      for (int h = 0; h < kNHit; ++h) {
+        const U4 w = philox4x32_10(U4{link, (uint32_t)h, sw_lo, sw_hi}, BC.seed, kKeyLLR);
+        const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
+        const uint32_t jp = (uint32_t)(npool_d * u1);
+        int32_t r1nc, m1nc;
+        sums(Lk.row(jp, P), r1nc, m1nc);
+        const double xc = A1q * ((double)r1nc - (double)r1c) + A2q * ((double)m1nc - (double)m1c);
+        const double M = margin + 0x1p-50 * fabs(xc);
+        // drive-in: axes 1, 2 accept sign dEw > 0; axis 3 accepts gx dS1 + gy dS2 < 0
+        int dec = drive ? (axis == 3 ? gt0_certified(-xc, M) : gt0_certified(xc, M)) : exp_gt_certified(xc, M, u2);
+        if (dec < 0) {  // decide exactly as the element path does
+          const uint32_t bnew = T.M(b, s_pool[jp]);
+          double e1 = 0, e2 = 0, e1n = 0, e2n = 0;
+#pragma unroll
+          for (int j = 0; j < NS; ++j) {
+            e1 += s_retr[T.M(b, stp[j])];
+            e1n += s_retr[T.M(bnew, stp[j])];
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
+            e2 += s_retr[T.M(b, re[q])];
+            e2n += s_retr[T.M(bnew, re[q])];
+          }
+          const double dS1 = e1n - e1, dS2 = e2n - e2;
+          bool accept;
+          if (axis == 3) {
+            accept = drive ? gx * dS1 + gy * dS2 < 0.0 : dsa_exp(-a * dS1 - a2 * dS2) > u2;
+          } else {
+            const double dEw = axis == 1 ? dS1 : dS2, dEbg = axis == 1 ? dS2 : dS1;
+            accept = drive ? sign * dEw > 0.0 : dsa_exp(-a * dEw - bg * dEbg) > u2;
+          }
+          dec = accept ? 1 : 0;
+        }
+        if (dec) {
+          const uint32_t p = s_pool[jp];
+          b = T.M(b, p);
+#pragma unroll
+          for (int k = 0; k < NL; ++k) tt[k] = T.M(tt[k], p);
+          r1c = r1nc;
+          m1c = m1nc;
+          ++nacc;
+        }
       }
-      if (accept) {
-        b = bnew;
-        r1 = r1n;
-        m1 = m1n;
-        ++nacc;
+    } else {
+      for (int h = 0; h < kNHit; ++h) {
+        const U4 w = philox4x32_10(U4{link, (uint32_t)h, sw_lo, sw_hi}, BC.seed, kKeyLLR);
+        const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
+        const uint32_t bnew = T.M(b, s_pool[(uint32_t)(npool_d * u1)]);
+        double r1n = 0, m1n = 0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) r1n += s_retr[T.M(bnew, stp[j])];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) m1n += s_retr[T.M(bnew, re[q])];
+        const double dS1 = r1n - r1, dS2 = m1n - m1;
+        bool accept;
+        if (axis == 3) {
+          if (drive)
+            accept = gx * dS1 + gy * dS2 < 0.0;  // greedy move toward the cell centre
+          else
+            accept = dsa_exp(-a * dS1 - a2 * dS2) > u2;  // llr2d.cpp:211-212
+        } else {
+          const double dEw = axis == 1 ? dS1 : dS2;
+          const double dEbg = axis == 1 ? dS2 : dS1;
+          if (drive)
+            accept = sign * dEw > 0.0;  // greedy move toward the window (llr.cpp:208-209)
+          else
+            accept = dsa_exp(-a * dEw - bg * dEbg) > u2;  // llr.cpp:211-212
+        }
+        if (accept) {
+          b = bnew;
+          r1 = r1n;
+          m1 = m1n;
+          ++nacc;
+        }
       }
     }
     long long d1 = 0, d2 = 0;
@@ -195,22 +306,48 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
 }
 
 // One wave per chain, in chunks of 64 proposals held in registers (eight
-// chunks are loaded at a time so the loads overlap).  With inclusive prefix
-// sums of the energy changes, lane k knows the running energy if every
-// proposal from the chunk's first undecided one up to k is accepted; the
-// first lane whose rule fails is refused (its link restored), all lanes
-// before it are accepted, and the scan resumes after it.  This is the
-// sequential rule exactly: each decision sees the energy after all earlier
-// accepted proposals.  A chunk costs one step plus one per refusal.
+// chunks are loaded at a time so the loads overlap, each lane also holding
+// its link's address and old value).  With inclusive prefix sums of the
+// energy changes, lane k knows the running energy if every proposal from
+// the chunk's first undecided one up to k is accepted: the first lane whose
+// rule fails ends a run of acceptances.  From there, a refused proposal
+// leaves the energy unchanged, so lane k's energy if every proposal from the
+// failing one up to k-1 is refused is E + dE_k: the first lane whose rule
+// holds on that ends a run of refusals, whose lanes restore their links
+// together.  This is the sequential rule exactly: each decision sees the
+// energy after all earlier accepted proposals.  A chunk costs one step per
+// run of acceptances or refusals.
 constexpr int kDecideSlab = 8;
 
+// Inclusive prefix sums over the wave: DPP row shifts (1, 2, 4, 8) scan each
+// row of 16 lanes, then the three lower rows' totals are added by row.
+// Lanes shifted in from outside a row read 0.
+template <int CTRL>
+__device__ inline long long dpp_shift64(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)((unsigned long long)v >> 32), CTRL, 0xf, 0xf, false);
+  return (long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 __device__ inline long long scan_i64(long long v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const long long t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
-  return v;
+  v += dpp_shift64<0x111>(v);  // row_shr:1
+  v += dpp_shift64<0x112>(v);  // row_shr:2
+  v += dpp_shift64<0x114>(v);  // row_shr:4
+  v += dpp_shift64<0x118>(v);  // row_shr:8
+  const long long t0 = readlane64(v, 15), t1 = readlane64(v, 31), t2 = readlane64(v, 47);
+  const int row = lane >> 4;
+  return v + (row == 0 ? 0 : row == 1 ? t0 : row == 2 ? t0 + t1 : t0 + t1 + t2);
+}
+
+__device__ inline int scan_i32(int v, int lane) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  const int t0 = __builtin_amdgcn_readlane(v, 15), t1 = __builtin_amdgcn_readlane(v, 31),
+            t2 = __builtin_amdgcn_readlane(v, 47);
+  const int row = lane >> 4;
+  return v + (row == 0 ? 0 : row == 1 ? t0 : row == 2 ? t0 + t1 : t0 + t1 + t2);
 }
 
 __global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
@@ -224,41 +361,51 @@ __global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrSt
   long long E1 = S.S1, E2 = S.S2;
   unsigned long long acc = 0;
   const Geom &G = LC.base.G;
+  uint16_t *links = LC.base.links;
   for (uint32_t slab = 0; slab < pr.count; slab += 64 * kDecideSlab) {
     long long r1[kDecideSlab], r2[kDecideSlab];
-    uint32_t rn[kDecideSlab];
+    uint32_t rn[kDecideSlab], pos[kDecideSlab];
+    uint16_t old[kDecideSlab];
 #pragma unroll
     for (int q = 0; q < kDecideSlab; ++q) {
       const uint32_t j = slab + 64u * q + (uint32_t)lane;
       const bool valid = j < pr.count;
-      r1[q] = valid ? LC.dE1[pr.offset + j] : 0;
-      r2[q] = valid ? LC.dE2[pr.offset + j] : 0;
-      rn[q] = valid ? LC.nacc[pr.offset + j] : 0;
+      const uint32_t idx = pr.offset + (valid ? j : 0);
+      r1[q] = valid ? LC.dE1[idx] : 0;
+      r2[q] = valid ? LC.dE2[idx] : 0;
+      rn[q] = valid ? LC.nacc[idx] : 0;
+      old[q] = LC.old[idx];
+      const uint32_t site = LC.base.sites[idx];
+      pos[q] = pr.mu * G.V + lpos(G, site, site_parity(G, site));
     }
 #pragma unroll
     for (int q = 0; q < kDecideSlab; ++q) {
       const uint32_t base = slab + 64u * q;
       if (base >= pr.count) break;
       const uint32_t n = pr.count - base < 64 ? pr.count - base : 64;
+      const bool mine = (uint32_t)lane < n;
       const long long P1 = scan_i64(r1[q], lane), P2 = scan_i64(r2[q], lane);
-      const long long PN = scan_i64((long long)rn[q], lane);
+      const int PN = scan_i32((int)rn[q], lane);
       for (uint32_t s0 = 0; s0 < n;) {  // s0 is uniform over the wave
-        const long long N1 = E1 + P1 - (s0 ? readlane64(P1, s0 - 1) : 0);
-        const long long N2 = E2 + P2 - (s0 ? readlane64(P2, s0 - 1) : 0);
-        const bool ok = (uint32_t)lane < s0 || (uint32_t)lane >= n || R(N1 - r1[q], N2 - r2[q], N1, N2);
+        // a run of acceptances [s0, f)
+        const long long B1 = s0 ? readlane64(P1, s0 - 1) : 0, B2 = s0 ? readlane64(P2, s0 - 1) : 0;
+        const long long N1 = E1 + P1 - B1, N2 = E2 + P2 - B2;
+        const bool ok = (uint32_t)lane < s0 || !mine || R(N1 - r1[q], N2 - r2[q], N1, N2);
         const unsigned long long bad = __ballot(!ok);
         const uint32_t f = bad ? (uint32_t)__builtin_ctzll(bad) : n;
-        if (f > s0) {  // accept [s0, f)
-          E1 += readlane64(P1, f - 1) - (s0 ? readlane64(P1, s0 - 1) : 0);
-          E2 += readlane64(P2, f - 1) - (s0 ? readlane64(P2, s0 - 1) : 0);
-          acc += (unsigned long long)(readlane64(PN, f - 1) - (s0 ? readlane64(PN, s0 - 1) : 0));
+        if (f > s0) {
+          E1 += readlane64(P1, f - 1) - B1;
+          E2 += readlane64(P2, f - 1) - B2;
+          acc += (unsigned long long)(__builtin_amdgcn_readlane(PN, (int)(f - 1)) -
+                                      (s0 ? __builtin_amdgcn_readlane(PN, (int)(s0 - 1)) : 0));
         }
-        if (f < n && (uint32_t)lane == f) {
-          const uint32_t idx = pr.offset + base + f;
-          const uint32_t site = LC.base.sites[idx];
-          LC.base.links[(size_t)pr.mu * G.V + lpos(G, site, site_parity(G, site))] = LC.old[idx];
-        }
-        s0 = f + 1;
+        if (f >= n) break;
+        // a run of refusals [f, g): lane f fails on E + dE_f by construction
+        const bool good = (uint32_t)lane > f && mine && R(E1, E2, E1 + r1[q], E2 + r2[q]);
+        const unsigned long long gm = __ballot(good);
+        const uint32_t g = gm ? (uint32_t)__builtin_ctzll(gm) : n;
+        if ((uint32_t)lane >= f && (uint32_t)lane < g) links[pos[q]] = old[q];
+        s0 = g;
       }
     }
   }

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <string>
 #include <thread>
@@ -63,7 +64,9 @@ struct dsa_llr {
   size_t tasks_cap = 0, obs_cap = 0;
   std::vector<uint32_t> pass_off, pass_n;
   uint32_t nobs = 0;
-  size_t lds = 0;
+  size_t lds = 0, lds_max = 0;
+  int kernel = DSA_KERNEL_AUTO;  // for chains added from now on
+  double margin_scale = 1.0;
   bool dirty = true;
   std::vector<dsa::LlrState> host_state;
 };
@@ -176,6 +179,10 @@ int dsa_llr_create(uint32_t D, uint32_t max_chains, uint64_t hbm_budget, dsa_llr
     return rc;
   };
   if (hipGetDevice(&L->device) != hipSuccess) return bail(fail(DSA_E_DEVICE, "hipGetDevice failed"));
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, L->device) != hipSuccess)
+    lds_max = 0;
+  L->lds_max = (size_t)lds_max;
   if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&L->d_chains, max_chains * sizeof(dsa::LlrChain)) != hipSuccess ||
       hipMalloc(&L->d_state, max_chains * sizeof(dsa::LlrState)) != hipSuccess)
@@ -226,7 +233,12 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   s.bytes = bytes;
   if (int rc = dsa::upload_tables(g, L->device, s.dt)) return rc;
   const dsa::Tables T = dsa::tables_view(g, s.dt);
-  if (dsa::sweep_lds_bytes(T) > 65536) return fail(DSA_E_RANGE, "group order too large for the LDS tables");
+  if (dsa::llr_lds_bytes(T, false) > 65536) return fail(DSA_E_RANGE, "group order too large for the LDS tables");
+  // the class path where the byte class tables exist and fit beside the
+  // static class values (256 B)
+  const bool cls = L->kernel != DSA_KERNEL_TABLE && T.cprod && dsa::llr_lds_bytes(T, true) + 256 <= L->lds_max;
+  if ((L->kernel == DSA_KERNEL_CLASS || L->kernel == DSA_KERNEL_CLASS_NARROW) && !cls)
+    return fail(DSA_E_STATE, "no class tables for this group on this device");
   std::string err = dsa::build_colour_schedule(s.G, s.S);
   if (!err.empty()) return fail(DSA_E_STATE, err);
   const size_t nl = (size_t)s.G.V * s.G.D;
@@ -296,6 +308,10 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   d.tol = p->tol;
   d.cap = p->cap ? p->cap : 20000;
   d.npass = (uint32_t)s.S.passes.size();
+  d.cls = cls ? 1 : 0;
+  d.cmscale = L->margin_scale;
+  if (cls)  // byte tables: class values in units of 2^-24, half a unit of quantisation
+    dsa::class_margin_terms(g->g, (int)p->D, 0x1p-25, d.ce1, d.ce2, d.cf1, d.cf2, d.cr1, d.cr2);
   dsa::LlrState st{};
   st.a = p->a0;
   st.a2 = p->axis == 3 ? p->a20 : 0.0;
@@ -307,9 +323,21 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   s.used = true;
   L->slots[c] = s;
   L->used_bytes += bytes;
-  L->lds = std::max(L->lds, dsa::sweep_lds_bytes(T));
+  L->lds = std::max(L->lds, dsa::llr_lds_bytes(T, cls));
   L->dirty = true;
   if (slot_out) *slot_out = c;
+  return 0;
+}
+
+int dsa_llr_set_kernel(dsa_llr *L, int32_t kernel, double margin_scale) {
+  if (!L) return fail(DSA_E_ARG, "dsa_llr_set_kernel: NULL engine");
+  if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS &&
+      kernel != DSA_KERNEL_CLASS_NARROW)
+    return fail(DSA_E_ARG, "dsa_llr_set_kernel: unknown kernel " + std::to_string(kernel));
+  if (!(margin_scale >= 1.0) || !std::isfinite(margin_scale))
+    return fail(DSA_E_ARG, "dsa_llr_set_kernel: margin_scale must be finite and >= 1");
+  L->kernel = kernel;
+  L->margin_scale = margin_scale;
   return 0;
 }
 

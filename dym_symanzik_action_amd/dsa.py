@@ -86,6 +86,8 @@ _SIGS = {
     "dsa_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "dsa_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(ChainParams), C.POINTER(C.c_uint32)]),
     "dsa_batch_finalize": (C.c_int, [C.c_void_p]),
+    "dsa_batch_set_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "dsa_batch_exact_decisions": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "dsa_batch_destroy": (None, [C.c_void_p]),
     "dsa_batch_sweep": (C.c_int, [C.c_void_p, C.c_uint32]),
     "dsa_batch_measure": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -376,6 +378,16 @@ class Batch:
         self._groups.append(group)
         return idx.value
 
+    def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
+        """Before finalize: "table", "class" / "class-narrow" or "auto" (the
+        class update where it fits); identical results."""
+        _check(lib().dsa_batch_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
+
+    def exact_decisions(self) -> int:
+        n = C.c_uint64()
+        _check(lib().dsa_batch_exact_decisions(self._h, C.byref(n)))
+        return n.value
+
     def finalize(self) -> "Batch":
         _check(lib().dsa_batch_finalize(self._h))
         self.final = True
@@ -487,6 +499,7 @@ class LlrEngine:
             L.dsa_llr_get_links.argtypes = [vp, C.c_uint32, vp]
             L.dsa_llr_footprint.argtypes = [C.POINTER(LlrParams), C.POINTER(C.c_uint64)]
             L.dsa_llr_serve.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]
+            L.dsa_llr_set_kernel.argtypes = [vp, C.c_int32, C.c_double]
             L.dsa_llr_create._sig = True
         h = C.c_void_p()
         _check(L.dsa_llr_create(D, max_chains, hbm_budget, C.byref(h)))
@@ -494,6 +507,13 @@ class LlrEngine:
         self.D = D
         self._shapes = {}
         self._keep = []
+
+    def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
+        """Pass kernel of chains added from now on: "table" (element values),
+        "class" / "class-narrow" (byte class tables in LDS, certified) or
+        "auto"; identical results.  A huge margin_scale decides every hit from
+        element values."""
+        _check(lib().dsa_llr_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
 
     def add(self, group: Group, params: LlrParams, links=None) -> int:
         slot = C.c_uint32()

@@ -166,6 +166,16 @@ typedef struct dsa_batch dsa_batch;
 int dsa_batch_create(dsa_batch **out);
 int dsa_batch_add(dsa_batch *b, const dsa_group *g, const dsa_chain_params *p, uint32_t *index);
 int dsa_batch_finalize(dsa_batch *b);
+/* Before finalize: the sweep kernel of the batch's chains (no counterpart in
+ * the reference; identical results either way).  TABLE: element values
+ * (dym-mod-metro.cpp:164-192).  CLASS / CLASS_NARROW: the byte-index class
+ * update of DSA_KERNEL_CLASS_NARROW for every chain (an error at finalize
+ * for a group without class tables that fit).  AUTO (default): that update
+ * for the chains where it fits and its decision bound is below 1e-3, the
+ * table update for the others. */
+int dsa_batch_set_kernel(dsa_batch *b, int32_t kernel, double margin_scale);
+/* Hits of the batch's class updates re-decided from element values. */
+int dsa_batch_exact_decisions(dsa_batch *b, uint64_t *n);
 void dsa_batch_destroy(dsa_batch *b);
 /* n sweeps of every chain (update(), dym-mod-metro.cpp:146-197). */
 int dsa_batch_sweep(dsa_batch *b, uint32_t n);
@@ -235,6 +245,15 @@ int dsa_llr_footprint(const dsa_llr_params *p, uint64_t *bytes);
  * given configuration, as run_pass does from the previous window (llr.cpp:256-274). */
 int dsa_llr_add(dsa_llr *L, const dsa_group *g, const dsa_llr_params *p, const int32_t *init_links, uint32_t *slot);
 int dsa_llr_remove(dsa_llr *L, uint32_t slot);
+/* Pass kernel for chains added from now on (no counterpart in the reference;
+ * results are identical either way).  TABLE: element ReTr values through L2
+ * gathers, as update_constrained sums them (llr.cpp:178-219).  CLASS or
+ * CLASS_NARROW: the byte class tables in LDS with certified decisions, each
+ * undecided hit decided again from element values; an error when the group
+ * has no class tables that fit.  AUTO (default): the class path where it
+ * fits.  margin_scale >= 1 widens the bound (a huge value decides every hit
+ * from element values). */
+int dsa_llr_set_kernel(dsa_llr *L, int32_t kernel, double margin_scale);
 /* nsweeps sweeps of every active chain (each in its own phase). */
 int dsa_llr_step(dsa_llr *L, uint32_t nsweeps);
 int dsa_llr_status_get(dsa_llr *L, uint32_t slot, dsa_llr_status *out);

@@ -26,18 +26,26 @@ MIX = [
 ]
 
 
-def build(dsa, dgroups, mix):
+def build(dsa, dgroups, mix, kernel="auto", margin_scale=1.0):
     b = dsa.Batch()
+    b.set_kernel(kernel, margin_scale)
     for name, D, Nt, Nx, b0, b1, b2, seed, start, rng in mix:
         b.add(dgroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng)
     return b.finalize()
 
 
-def test_batch_equals_lone_chains(dsa, dgroups):
-    say("case batch vs lone chains")
+@pytest.mark.parametrize("kernel", [("auto", 1.0), ("table", 1.0), ("auto", 1e300)],
+                         ids=["auto", "table", "class-all-exact"])
+def test_batch_equals_lone_chains(dsa, dgroups, kernel):
+    """Each chain equals the chain alone, with the class update where it fits
+    (auto), the table update everywhere, and the class update deciding every
+    hit from element values."""
+    say("case batch vs lone chains", kernel)
     nsw = 3
-    with build(dsa, dgroups, MIX) as b:
+    with build(dsa, dgroups, MIX, *kernel) as b:
         b.sweep(nsw)
+        if kernel[1] > 1:
+            assert b.exact_decisions() > 0
         hits, accs = b.counters()
         for c, (name, D, Nt, Nx, b0, b1, b2, seed, start, rng) in enumerate(MIX):
             with dsa.Chain(dgroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng) as ch:
