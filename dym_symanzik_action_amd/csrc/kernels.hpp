@@ -113,7 +113,8 @@ struct LlrChain {
   uint16_t *old;     // previous link values of the current pass, indexed like sites
   long long *dE1;    // per link: proposed change of S1 (fixed point), indexed like sites
   long long *dE2;    // per link: proposed change of S2
-  uint8_t *nacc;     // per link: accepted hits of its proposal
+  uint8_t *nacc;     // per link: accepted hits of its proposal, + 0x80 if it moved the link
+  const uint32_t *pos;  // per link: its index in the links array, indexed like sites
   const PassRange *passes;
   int axis;          // 1: window S1 (plaquettes), 2: window S2 (rectangles), 3: box on both
   double bg;         // coupling of the other action (axes 1, 2)

@@ -38,6 +38,8 @@ __device__ inline long long readlane64(long long v, uint32_t k) {
 
 __device__ inline bool llr_active(uint32_t phase) { return phase == kLlrSeed || phase == kLlrRM; }
 
+constexpr int kChanged = 0x80;  // flag in a proposal's hit count: it moved its link
+
 __device__ inline bool llr_inside(const LlrChain &LC, long long E1, long long E2) {
   if (LC.axis == 3) return E1 >= LC.lo && E1 <= LC.hi && E2 >= LC.lo2 && E2 <= LC.hi2;
   const long long Ew = LC.axis == 1 ? E1 : E2;
@@ -301,23 +303,28 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     LC.old[idx] = (uint16_t)b0;
     LC.dE1[idx] = d1;
     LC.dE2[idx] = d2;
-    LC.nacc[idx] = (uint8_t)nacc;
+    LC.nacc[idx] = (uint8_t)(nacc | (b != b0 ? (uint32_t)kChanged : 0u));
   }
 }
 
-// One wave per chain, in chunks of 64 proposals held in registers (eight
-// chunks are loaded at a time so the loads overlap, each lane also holding
-// its link's address and old value).  With inclusive prefix sums of the
-// energy changes, lane k knows the running energy if every proposal from
-// the chunk's first undecided one up to k is accepted: the first lane whose
-// rule fails ends a run of acceptances.  From there, a refused proposal
-// leaves the energy unchanged, so lane k's energy if every proposal from the
-// failing one up to k-1 is refused is E + dE_k: the first lane whose rule
-// holds on that ends a run of refusals, whose lanes restore their links
-// together.  This is the sequential rule exactly: each decision sees the
-// energy after all earlier accepted proposals.  A chunk costs one step per
-// run of acceptances or refusals.
-constexpr int kDecideSlab = 8;
+// A proposal that left its link as it was changes no energy, and its rule
+// does not depend on where it falls: with the window rule the energy is
+// inside the window when it is met (accepted), with the drive-in rule it
+// does not move closer (refused).  Such proposals are settled apart and do
+// not break the runs below.
+//
+// One block of kDecide lanes per chain, a segment of kDecide proposals at a
+// time.  Block-wide inclusive prefix sums of the energy changes tell lane k
+// the running energy if every proposal from the first undecided one up to k
+// is accepted: the first lane whose rule fails ends a run of acceptances.
+// From there a refused proposal leaves the energy unchanged, so lane k's
+// energy if every proposal from the failing one up to k-1 is refused is
+// E + dE_k: the first lane whose rule holds on that ends a run of refusals,
+// whose lanes restore their links together.  This is the sequential rule
+// exactly: each decision sees the energy after all earlier accepted
+// proposals.  A segment costs one scan plus two block-wide minima per run.
+constexpr int kDecide = 1024;
+constexpr int kDecideWaves = kDecide / 64;
 
 // Inclusive prefix sums over the wave: DPP row shifts (1, 2, 4, 8) scan each
 // row of 16 lanes, then the three lower rows' totals are added by row.
@@ -350,66 +357,131 @@ __device__ inline int scan_i32(int v, int lane) {
   return v + (row == 0 ? 0 : row == 1 ? t0 : row == 2 ? t0 + t1 : t0 + t1 + t2);
 }
 
-__global__ __launch_bounds__(64) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
+// The smallest lane index whose pred holds (or n).  Three slots rotate: the
+// slot of call k is reset after call k-2's readers have all passed call k-1's
+// barrier, and before call k's writers (they pass call k-1's barrier after
+// the reset in call k-1).
+__device__ inline uint32_t block_first(bool pred, uint32_t n, unsigned *slots, uint32_t &k) {
+  const unsigned long long bm = __ballot(pred);
+  if ((threadIdx.x & 63u) == 0 && bm) atomicMin(&slots[k], (threadIdx.x & ~63u) + (unsigned)__builtin_ctzll(bm));
+  __syncthreads();
+  const unsigned r = slots[k];
+  const uint32_t nx = k == 2 ? 0 : k + 1;
+  if (threadIdx.x == 0) slots[nx == 2 ? 0 : nx + 1] = 0xffffffffu;
+  k = nx;
+  return r < n ? r : n;
+}
+
+__global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
+  __shared__ long long sP1[kDecide], sP2[kDecide];
+  __shared__ int sPN[kDecide];
+  __shared__ uint32_t sPos[kDecide];
+  __shared__ uint16_t sOld[kDecide];
+  __shared__ uint8_t sChg[kDecide];
+  __shared__ long long sW1[kDecideWaves], sW2[kDecideWaves];
+  __shared__ int sWN[kDecideWaves], sWT[kDecideWaves];
+  __shared__ unsigned slots[3];
+  __shared__ long long sE[2];
+  __shared__ unsigned long long sAcc;
   const uint32_t c = blockIdx.x;
-  const int lane = (int)threadIdx.x;
+  const uint32_t tid = threadIdx.x;
+  const int lane = (int)(tid & 63u), wv = (int)(tid >> 6);
   const LlrChain &LC = chains[c];
   LlrState &S = st[c];
-  if (!llr_active(S.phase) || pass >= LC.npass) return;
+  if (!llr_active(S.phase) || pass >= LC.npass) return;  // uniform over the block
   const PassRange pr = LC.passes[pass];
   const LlrRule R = llr_rule_of(LC, S);
-  long long E1 = S.S1, E2 = S.S2;
+  long long E1 = S.S1, E2 = S.S2;  // uniform over the block
   unsigned long long acc = 0;
-  const Geom &G = LC.base.G;
   uint16_t *links = LC.base.links;
-  for (uint32_t slab = 0; slab < pr.count; slab += 64 * kDecideSlab) {
-    long long r1[kDecideSlab], r2[kDecideSlab];
-    uint32_t rn[kDecideSlab], pos[kDecideSlab];
-    uint16_t old[kDecideSlab];
-#pragma unroll
-    for (int q = 0; q < kDecideSlab; ++q) {
-      const uint32_t j = slab + 64u * q + (uint32_t)lane;
-      const bool valid = j < pr.count;
-      const uint32_t idx = pr.offset + (valid ? j : 0);
-      r1[q] = valid ? LC.dE1[idx] : 0;
-      r2[q] = valid ? LC.dE2[idx] : 0;
-      rn[q] = valid ? LC.nacc[idx] : 0;
-      old[q] = LC.old[idx];
-      const uint32_t site = LC.base.sites[idx];
-      pos[q] = pr.mu * G.V + lpos(G, site, site_parity(G, site));
+  if (tid < 3) slots[tid] = 0xffffffffu;
+  uint32_t k = 0;
+  for (uint32_t seg = 0; seg < pr.count; seg += kDecide) {
+    const uint32_t n = pr.count - seg < (uint32_t)kDecide ? pr.count - seg : (uint32_t)kDecide;
+    const bool mine = tid < n;
+    const uint32_t idx = pr.offset + seg + (mine ? tid : 0);
+    const long long d1 = mine ? LC.dE1[idx] : 0, d2 = mine ? LC.dE2[idx] : 0;
+    const int raw = mine ? (int)LC.nacc[idx] : 0;
+    const bool changed = (raw & kChanged) != 0;
+    const int dn = changed ? raw & ~kChanged : 0;  // hits of proposals that moved their link
+    long long p1 = scan_i64(d1, lane), p2 = scan_i64(d2, lane);
+    int pn = scan_i32(dn, lane);
+    const int pt = scan_i32(changed ? 0 : raw, lane);  // hits of the others
+    if (lane == 63) {
+      sW1[wv] = p1;
+      sW2[wv] = p2;
+      sWN[wv] = pn;
+      sWT[wv] = pt;
     }
-#pragma unroll
-    for (int q = 0; q < kDecideSlab; ++q) {
-      const uint32_t base = slab + 64u * q;
-      if (base >= pr.count) break;
-      const uint32_t n = pr.count - base < 64 ? pr.count - base : 64;
-      const bool mine = (uint32_t)lane < n;
-      const long long P1 = scan_i64(r1[q], lane), P2 = scan_i64(r2[q], lane);
-      const int PN = scan_i32((int)rn[q], lane);
-      for (uint32_t s0 = 0; s0 < n;) {  // s0 is uniform over the wave
-        // a run of acceptances [s0, f)
-        const long long B1 = s0 ? readlane64(P1, s0 - 1) : 0, B2 = s0 ? readlane64(P2, s0 - 1) : 0;
-        const long long N1 = E1 + P1 - B1, N2 = E2 + P2 - B2;
-        const bool ok = (uint32_t)lane < s0 || !mine || R(N1 - r1[q], N2 - r2[q], N1, N2);
-        const unsigned long long bad = __ballot(!ok);
-        const uint32_t f = bad ? (uint32_t)__builtin_ctzll(bad) : n;
-        if (f > s0) {
-          E1 += readlane64(P1, f - 1) - B1;
-          E2 += readlane64(P2, f - 1) - B2;
-          acc += (unsigned long long)(__builtin_amdgcn_readlane(PN, (int)(f - 1)) -
-                                      (s0 ? __builtin_amdgcn_readlane(PN, (int)(s0 - 1)) : 0));
+    sOld[tid] = LC.old[idx];
+    sPos[tid] = LC.pos[idx];
+    sChg[tid] = changed ? 1 : 0;
+    __syncthreads();
+    for (int w = 0; w < wv; ++w) {
+      p1 += sW1[w];
+      p2 += sW2[w];
+      pn += sWN[w];
+    }
+    if (!R.drive) {  // unchanged links: all accepted under the window rule
+      int tot = 0;
+      for (int w = 0; w < kDecideWaves; ++w) tot += sWT[w];
+      acc += (unsigned long long)tot;
+    }
+    sP1[tid] = p1;
+    sP2[tid] = p2;
+    sPN[tid] = pn;
+    __syncthreads();
+    // the whole segment at once: the first proposal whose rule fails
+    const uint32_t f0 = block_first(changed && !R(E1 + p1 - d1, E2 + p2 - d2, E1 + p1, E2 + p2), n, slots, k);
+    if (f0 > 0) {
+      E1 += sP1[f0 - 1];
+      E2 += sP2[f0 - 1];
+      acc += (unsigned long long)sPN[f0 - 1];
+    }
+    if (f0 < n) {
+      // contested: one wave walks the rest by runs of acceptances and
+      // refusals, 64 proposals at a time, with no block barriers
+      if (wv == 0) {
+        for (uint32_t base = f0 & ~63u; base < n; base += 64) {
+          const uint32_t cn = n - base < 64 ? n - base : 64;
+          const uint32_t kk = base + (uint32_t)lane;
+          const bool valid = (uint32_t)lane < cn;
+          const long long q1 = valid ? sP1[kk] : 0, q2 = valid ? sP2[kk] : 0;
+          const long long e1 = valid ? q1 - (kk ? sP1[kk - 1] : 0) : 0, e2 = valid ? q2 - (kk ? sP2[kk - 1] : 0) : 0;
+          const bool chg = valid && sChg[kk];
+          uint32_t s0 = base < f0 ? f0 - base : 0;  // uniform over the wave
+          while (s0 < cn) {
+            const uint32_t at = base + s0;
+            const long long B1 = at ? sP1[at - 1] : 0, B2 = at ? sP2[at - 1] : 0;
+            const long long N1 = E1 + q1 - B1, N2 = E2 + q2 - B2;
+            const unsigned long long bm = __ballot(chg && (uint32_t)lane >= s0 && !R(N1 - e1, N2 - e2, N1, N2));
+            const uint32_t f = bm ? (uint32_t)__builtin_ctzll(bm) : cn;
+            if (f > s0) {  // accept [s0, f)
+              E1 += sP1[base + f - 1] - B1;
+              E2 += sP2[base + f - 1] - B2;
+              acc += (unsigned long long)(sPN[base + f - 1] - (at ? sPN[at - 1] : 0));
+            }
+            if (f >= cn) break;
+            const unsigned long long gm = __ballot(chg && (uint32_t)lane > f && R(E1, E2, E1 + e1, E2 + e2));
+            const uint32_t g = gm ? (uint32_t)__builtin_ctzll(gm) : cn;
+            if (chg && (uint32_t)lane >= f && (uint32_t)lane < g) links[sPos[kk]] = sOld[kk];
+            s0 = g;
+          }
         }
-        if (f >= n) break;
-        // a run of refusals [f, g): lane f fails on E + dE_f by construction
-        const bool good = (uint32_t)lane > f && mine && R(E1, E2, E1 + r1[q], E2 + r2[q]);
-        const unsigned long long gm = __ballot(good);
-        const uint32_t g = gm ? (uint32_t)__builtin_ctzll(gm) : n;
-        if ((uint32_t)lane >= f && (uint32_t)lane < g) links[pos[q]] = old[q];
-        s0 = g;
+        if (lane == 0) {
+          sE[0] = E1;
+          sE[1] = E2;
+          sAcc = acc;
+        }
       }
+      __syncthreads();
+      E1 = sE[0];
+      E2 = sE[1];
+      acc = sAcc;
     }
+    __syncthreads();  // the next segment overwrites the shared arrays
   }
-  if (lane == 0) {
+  if (tid == 0) {
     S.S1 = E1;
     S.S2 = E2;
     S.acc += acc;
@@ -537,7 +609,7 @@ hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const Ba
 
 hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, uint32_t n, uint32_t pass, hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_llr_decide, dim3(n), dim3(64), 0, s, chains, st, pass);
+  hipLaunchKernelGGL(k_llr_decide, dim3(n), dim3(kDecide), 0, s, chains, st, pass);
   return hipGetLastError();
 }
 

@@ -32,19 +32,20 @@ struct Slot {
   dsa::Schedule S;
   dsa::DeviceTables dt;
   uint16_t *links = nullptr, *old = nullptr;
-  uint32_t *sites = nullptr;
+  uint32_t *sites = nullptr, *pos = nullptr;
   long long *dE1 = nullptr, *dE2 = nullptr;
   uint8_t *nacc = nullptr;
   dsa::PassRange *passes = nullptr;
   uint64_t bytes = 0;
 };
 
-// links, previous values, site lists, two energy changes and a hit count per link
+// links, previous values, site lists, link positions, two energy changes and
+// a hit count per link
 uint64_t footprint(const dsa_llr_params &p) {
   uint64_t V = p.Nt;
   for (uint32_t d = 1; d < p.D; ++d) V *= p.Nx;
   const uint64_t nl = V * p.D;
-  return nl * (2 + 2 + 4 + 8 + 8 + 1) + 64 * sizeof(dsa::PassRange) + sizeof(dsa::LlrChain) +
+  return nl * (2 + 2 + 4 + 4 + 8 + 8 + 1) + 64 * sizeof(dsa::PassRange) + sizeof(dsa::LlrChain) +
          sizeof(dsa::LlrState);
 }
 
@@ -123,8 +124,8 @@ int rebuild_tasks(dsa_llr *L) {
 }
 
 void release(Slot &s) {
-  for (void *p : {(void *)s.links, (void *)s.old, (void *)s.sites, (void *)s.dE1, (void *)s.dE2, (void *)s.nacc,
-                  (void *)s.passes})
+  for (void *p : {(void *)s.links, (void *)s.old, (void *)s.sites, (void *)s.pos, (void *)s.dE1, (void *)s.dE2,
+                  (void *)s.nacc, (void *)s.passes})
     if (p) (void)hipFree(p);
   s = Slot();
 }
@@ -249,13 +250,28 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   const size_t ns = s.S.sites.size();
   std::vector<dsa::PassRange> pr;
   for (const dsa::Pass &ps : s.S.passes) pr.push_back(dsa::PassRange{ps.offset, ps.count, ps.mu, 0});
+  // index of each scheduled link in the per-direction, per-parity link array
+  std::vector<uint32_t> lpos(ns);
+  for (const dsa::Pass &ps : s.S.passes)
+    for (uint32_t q = ps.offset; q < ps.offset + ps.count; ++q) {
+      const uint32_t site = s.S.sites[q];
+      uint32_t par = 0, r = site;
+      for (int d = 0; d < s.G.D; ++d) {
+        par += r % s.G.ext[d];
+        r /= s.G.ext[d];
+      }
+      const uint32_t at = s.G.eo ? ((par & 1u) ? s.G.halfV : 0u) + (site >> 1) : site;
+      lpos[q] = (uint32_t)ps.mu * s.G.V + at;
+    }
   if (hipMalloc(&s.links, nl * sizeof(uint16_t)) != hipSuccess || hipMalloc(&s.old, ns * sizeof(uint16_t)) != hipSuccess ||
-      hipMalloc(&s.sites, ns * sizeof(uint32_t)) != hipSuccess || hipMalloc(&s.dE1, ns * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&s.sites, ns * sizeof(uint32_t)) != hipSuccess || hipMalloc(&s.pos, ns * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&s.dE1, ns * sizeof(long long)) != hipSuccess ||
       hipMalloc(&s.dE2, ns * sizeof(long long)) != hipSuccess || hipMalloc(&s.nacc, ns) != hipSuccess ||
       hipMalloc(&s.passes, pr.size() * sizeof(dsa::PassRange)) != hipSuccess)
     return cleanup(fail(DSA_E_DEVICE, "chain allocation failed"));
   if (hipMemcpyAsync(s.sites, s.S.sites.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, L->stream) !=
           hipSuccess ||
+      hipMemcpyAsync(s.pos, lpos.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, L->stream) != hipSuccess ||
       hipMemcpyAsync(s.passes, pr.data(), pr.size() * sizeof(dsa::PassRange), hipMemcpyHostToDevice, L->stream) !=
           hipSuccess)
     return cleanup(fail(DSA_E_DEVICE, "upload failed"));
@@ -288,6 +304,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   d.dE1 = s.dE1;
   d.dE2 = s.dE2;
   d.nacc = s.nacc;
+  d.pos = s.pos;
   d.passes = s.passes;
   d.axis = p->axis;
   d.bg = p->bg;
