@@ -12,7 +12,7 @@ Prints one JSON line.  Besides the contract fields it carries
                 traffic per link-update), with per-launch times from HIP
                 events on the chain's stream, and its random-gather ceiling
                 (table gathers per link-update vs a measured probe: 608 for
-                the table kernel; 108 + 25 per accepted hit + 98 per exact
+                the table kernel; 85 + 25 per accepted hit + 98 per exact
                 re-decision for the class kernel);
   cpu_baseline  the reference binary (oracle/_ref/dym-mod-metro, built from
                 the reference's sources) timed on this host's cores for two
@@ -39,7 +39,7 @@ SEED = 11
 NHIT = 20
 BYTES_PER_LINK_UPDATE = 66      # 64 B of link reads + 2 B write per updated link
 GATHERS_TABLE = 608     # 84 staple + 24 initial + 20*25 hit-loop table gathers
-GATHERS_CLASS = 108     # 84 staple + 24 staple products with the link; per hit: LDS only
+GATHERS_CLASS = 85      # the 24 staple-times-link products, built directly; per hit: LDS only
 GATHERS_CLASS_ACCEPT = 25   # new link + 24 staple products moved by the accepted factor
 GATHERS_CLASS_EXACT = 98    # b', b^-1 and 4 per staple for an exact re-decision
 KERNEL_NAME = {"table": "k_sweep<4>", "class": "k_sweep_cls<4, true>", "class-narrow": "k_sweep_cls<4, false>"}
