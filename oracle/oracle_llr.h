@@ -6,6 +6,11 @@
  *            minstd streams).
  * ogl_dev_*: the device LLR algorithm of this build (per-pass window test in
  *            fixed point, Philox streams), restated for bit-exact checks.
+ *            It is not the reference's algorithm, so it is pinned to the
+ *            reference statistically (a_n of the reference llr and both tilts
+ *            of the reference llr2d on their own fixtures, tests/test_oracle.py)
+ *            and by invariants (incremental energies equal a recomputation,
+ *            the window holds once reached).
  */
 #ifndef DSA_ORACLE_LLR_H
 #define DSA_ORACLE_LLR_H

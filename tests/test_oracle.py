@@ -353,3 +353,66 @@ def test_device_llr_restatement_keeps_energies_consistent(oracle, ogroups):
     assert s["phase"] == 2 and s["iterations"] == 4 and s["reached"] == 1
     assert s["maxdrift"] < 1e-6
     assert len(seen) > 10
+
+
+def _run_llr_chains(oracle, jobs):
+    """Finish each oracle LLR chain; ctypes releases the GIL, so threads run them in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(job):
+        L = oracle.OLlr(*job[0], **job[1])
+        for _ in range(400):
+            if L.status()["phase"] == 2:
+                break
+            L.step(25)
+        return L.status()
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        return list(ex.map(one, jobs))
+
+
+@pytest.mark.slow
+def test_device_llr_restatement_agrees_with_reference_llr(oracle, ogroups):
+    """The device algorithm as the oracle restates it (per-pass proposals,
+    ordered window acceptance, fixed point, Philox) against the reference llr
+    binary on its own command line (tests/golden/llr_stats_4x4.txt: 4^4,
+    window -1500 +- 75, bg -0.1, a0 1.5, K 20, NRM 30, 8 hot + 8 cold): the
+    mean a_n within 3.5 standard errors."""
+    ref = [l for l in open(os.path.join(GOLDEN, "llr_stats_4x4.txt")) if l.startswith("ANE:")][0].split()
+    m_ref, sd_ref, n_ref = float(ref[2]), float(ref[3]), int(ref[4])
+    g = ogroups("mys1080-v4")
+    jobs = [((g, 4, 4, 4, 1, -0.1, -1500.0, 75.0, 1.5, 900 + k), dict(start="hot" if k < 8 else "cold", K=20, NRM=30))
+            for k in range(16)]
+    st = _run_llr_chains(oracle, jobs)
+    assert all(s["reached"] == 1 and s["iterations"] == 30 for s in st)
+    a = np.array([s["a"] for s in st])
+    z = abs(a.mean() - m_ref) / np.hypot(sd_ref / np.sqrt(n_ref), a.std(ddof=1) / np.sqrt(a.size))
+    assert z < 3.5, (a.mean(), m_ref, z)
+
+
+@pytest.mark.slow
+def test_device_llr2d_restatement_agrees_with_reference_llr2d(oracle, ogroups):
+    """The oracle's box-window (llr2d) restatement from the reference's own
+    seeded cells (tests/golden/llr2d_seed_3x3.npz, 3^4, K 20, NRM 30, a0 = 1):
+    both tilts of every cell within 4 standard errors of the reference's eight
+    seeds."""
+    from dym_symanzik_action_amd import cfgio
+
+    f = np.load(os.path.join(GOLDEN, "llr2d_seed_3x3.npz"))
+    cells = cfgio.parse_manifest([str(l) for l in f["manifest"]])
+    g = ogroups("mys1080-v4")
+    nrep = 8
+    jobs = []
+    for k, c in enumerate(cells):
+        links = f[os.path.basename(c["path"])[:-4]]
+        for r in range(nrep):
+            jobs.append(((g, 4, 3, 3, 3, 0.0, c["E1"], c["hw1"], 1.0, 7000 + 100 * k + r),
+                         dict(K=20, NRM=30, Ewin2=c["E2"], hw2=c["hw2"], a20=1.0, links=links)))
+    st = _run_llr_chains(oracle, jobs)
+    assert all(s["reached"] == 1 and s["iterations"] == 30 for s in st)
+    a1 = np.array([s["a"] for s in st]).reshape(len(cells), nrep)
+    a2 = np.array([s["a2"] for s in st]).reshape(len(cells), nrep)
+    for k in range(len(cells)):
+        for mine, ref in ((a1[k], f["stats_a1"][:, k]), (a2[k], f["stats_a2"][:, k])):
+            se = np.hypot(mine.std(ddof=1) / np.sqrt(mine.size), ref.std(ddof=1) / np.sqrt(ref.size))
+            assert abs(mine.mean() - ref.mean()) < 4 * se, (k, mine.mean(), ref.mean(), se)
