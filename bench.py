@@ -122,7 +122,8 @@ def batched_vs_serial(dsa, sweeps):
     b = dsa.Batch()
     for name, (D, Nt, Nx), b1, b2, seed in chains:
         b.add(groups[name], D, Nt, Nx, 0.0, b1, b2, seed)
-    b.finalize()
+    b.finalize()  # times 256/512/1024 links per block and keeps the fastest
+    task_links, tune_ms = b.task_links()
     b.sweep(1)
     t0 = time.perf_counter()
     b.sweep(sweeps)
@@ -142,7 +143,8 @@ def batched_vs_serial(dsa, sweeps):
     return {"workload": "256 chains, 16x16 (beta1, beta2) grid, shapes 8^4/12^4/16^3x8/8^3x16 round-robin, "
                         "192 S(1080) + 64 S(216), hot seeds 100+i", "sweeps": sweeps, "links": links,
             "batched_link_updates_per_s": sweeps * links / tb, "serial_link_updates_per_s": sweeps * links / ts,
-            "batched_s": tb, "serial_s": ts, "speedup": ts / tb}
+            "batched_s": tb, "serial_s": ts, "speedup": ts / tb, "task_links": task_links,
+            "tuning_ms_per_sweep": {"256": tune_ms[0], "512": tune_ms[1], "1024": tune_ms[2]}}
 
 
 def serving_requests(dsa, nreq, seed=2024):

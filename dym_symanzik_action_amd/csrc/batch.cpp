@@ -23,7 +23,8 @@ using dsa::fail;
 
 namespace {
 
-constexpr uint32_t kTaskLinks = 256;   // links per sweep block (one per lane)
+// links per sweep block (one per lane): the candidates the tuner times
+constexpr uint32_t kTaskCandidates[] = {256, 512, 1024};
 constexpr uint32_t kObsSites = 1024;   // sites per measurement block
 
 struct HostChain {
@@ -61,6 +62,9 @@ struct dsa_batch {
   size_t lds = 0;
   int kernel = DSA_KERNEL_AUTO;
   double margin_scale = 1.0;
+  uint32_t task_links = 0;  // requested links per block (0: tuned at finalize)
+  uint32_t block = 256;     // in use
+  std::vector<double> tune_ms;  // ms per sweep of each candidate, when tuned
   unsigned long long *d_fall = nullptr;
   uint64_t sweeps = 0;
   double last_ms = 0;
@@ -74,11 +78,98 @@ namespace {
 int enqueue_sweeps(dsa_batch *b, uint32_t n) {
   for (uint32_t it = 0; it < n; ++it) {
     for (size_t q = 0; q < b->pass_off.size(); ++q)
-      HIP_TRY(dsa::launch_sweep_batch(b->D, b->d_chains, b->d_tasks + b->pass_off[q], b->pass_n[q], b->lds,
-                                      b->d_sweep, b->d_acc, b->stream));
+      HIP_TRY(dsa::launch_sweep_batch(b->D, b->d_chains, b->d_tasks + b->pass_off[q], b->pass_n[q], b->block,
+                                      b->lds, b->d_sweep, b->d_acc, b->stream));
     HIP_TRY(dsa::launch_count_sweep(b->d_sweep, b->stream));
   }
   return 0;
+}
+
+// Pass q of every chain, split into blocks of `links` links of one chain,
+// uploaded to d_tasks (sized for the smallest candidate).
+int set_tasks(dsa_batch *b, uint32_t links) {
+  std::vector<dsa::BatchTask> tasks;
+  size_t maxpass = 0;
+  for (const HostChain &hc : b->chains) maxpass = std::max(maxpass, hc.S.passes.size());
+  b->pass_off.clear();
+  b->pass_n.clear();
+  for (size_t q = 0; q < maxpass; ++q) {
+    b->pass_off.push_back((uint32_t)tasks.size());
+    for (size_t c = 0; c < b->chains.size(); ++c) {
+      const HostChain &hc = b->chains[c];
+      if (q >= hc.S.passes.size()) continue;
+      const dsa::Pass &ps = hc.S.passes[q];
+      for (uint32_t o = 0; o < ps.count; o += links)
+        tasks.push_back(dsa::BatchTask{(uint32_t)c, ps.offset + o, std::min(links, ps.count - o), ps.mu});
+    }
+    b->pass_n.push_back((uint32_t)tasks.size() - b->pass_off.back());
+  }
+  if (!tasks.empty())
+    HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(dsa::BatchTask), hipMemcpyHostToDevice));
+  b->block = links;
+  return 0;
+}
+
+// Times two sweeps with each block size (after one untimed sweep) and keeps
+// the fastest.  The state the sweeps change (links, minstd states, counters)
+// is saved before and restored after each candidate, so tuning leaves no
+// trace in the results.
+int tune_tasks(dsa_batch *b, size_t nlinks, size_t nms) {
+  const size_t nc = b->chains.size();
+  uint16_t *links0 = nullptr;
+  uint32_t *ms0 = nullptr;
+  unsigned long long *acc0 = nullptr, *misc0 = nullptr;
+  auto release = [&]() {
+    for (void *p : {(void *)links0, (void *)ms0, (void *)acc0, (void *)misc0})
+      if (p) (void)hipFree(p);
+  };
+  auto copy = [&](void *dst, const void *src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b->stream) : hipSuccess;
+  };
+  hipError_t e = hipMalloc(&links0, nlinks * sizeof(uint16_t));
+  if (e == hipSuccess && nms) e = hipMalloc(&ms0, nms * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&acc0, nc * sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&misc0, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = copy(links0, b->d_links, nlinks * sizeof(uint16_t));
+  if (e == hipSuccess) e = copy(ms0, b->d_mstate, nms * sizeof(uint32_t));
+  if (e == hipSuccess) e = copy(acc0, b->d_acc, nc * sizeof(unsigned long long));
+  if (e == hipSuccess) e = copy(misc0, b->d_sweep, sizeof(unsigned long long));
+  if (e == hipSuccess) e = copy(misc0 + 1, b->d_fall, sizeof(unsigned long long));
+  if (e != hipSuccess) {
+    release();
+    return fail(DSA_E_DEVICE, std::string("batch tuning: ") + hipGetErrorString(e));
+  }
+  uint32_t best = kTaskCandidates[0];
+  double best_ms = 0;
+  b->tune_ms.clear();
+  for (uint32_t links : kTaskCandidates) {
+    int rc = set_tasks(b, links);
+    if (!rc) rc = enqueue_sweeps(b, 1);
+    if (!rc && hipEventRecord(b->ev0, b->stream) != hipSuccess) rc = fail(DSA_E_DEVICE, "event record failed");
+    if (!rc) rc = enqueue_sweeps(b, 2);
+    float ms = 0;
+    if (!rc && (hipEventRecord(b->ev1, b->stream) != hipSuccess || hipEventSynchronize(b->ev1) != hipSuccess ||
+                hipEventElapsedTime(&ms, b->ev0, b->ev1) != hipSuccess))
+      rc = fail(DSA_E_DEVICE, "event timing failed");
+    e = copy(b->d_links, links0, nlinks * sizeof(uint16_t));
+    if (e == hipSuccess) e = copy(b->d_mstate, ms0, nms * sizeof(uint32_t));
+    if (e == hipSuccess) e = copy(b->d_acc, acc0, nc * sizeof(unsigned long long));
+    if (e == hipSuccess) e = copy(b->d_sweep, misc0, sizeof(unsigned long long));
+    if (e == hipSuccess) e = copy(b->d_fall, misc0 + 1, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (!rc && e != hipSuccess) rc = fail(DSA_E_DEVICE, std::string("batch tuning: ") + hipGetErrorString(e));
+    if (rc) {
+      release();
+      return rc;
+    }
+    b->tune_ms.push_back(ms / 2.0);
+    if (b->tune_ms.size() == 1 || ms / 2.0 < best_ms) {
+      best = links;
+      best_ms = ms / 2.0;
+    }
+  }
+  release();
+  return set_tasks(b, best);
 }
 
 int enqueue_measure(dsa_batch *b) {
@@ -205,34 +296,43 @@ int dsa_batch_finalize(dsa_batch *b) {
       obs.push_back(dsa::ObsTask{(uint32_t)c, s0, std::min(kObsSites, hc.G.V - s0)});
     on[c] = (uint32_t)obs.size() - ofirst[c];
   }
-  // pass q of every chain, split into blocks of kTaskLinks links of one chain
-  std::vector<dsa::BatchTask> tasks;
-  for (size_t q = 0; q < maxpass; ++q) {
-    b->pass_off.push_back((uint32_t)tasks.size());
-    for (size_t c = 0; c < nc; ++c) {
-      const HostChain &hc = b->chains[c];
-      if (q >= hc.S.passes.size()) continue;
-      const dsa::Pass &ps = hc.S.passes[q];
-      for (uint32_t o = 0; o < ps.count; o += kTaskLinks)
-        tasks.push_back(dsa::BatchTask{(uint32_t)c, ps.offset + o, std::min(kTaskLinks, ps.count - o), ps.mu});
-    }
-    b->pass_n.push_back((uint32_t)tasks.size() - b->pass_off.back());
-  }
+  // tasks: at most one per kTaskCandidates[0] links of every pass of every chain
+  size_t ntasks_max = 0;
+  for (const HostChain &hc : b->chains)
+    for (const dsa::Pass &ps : hc.S.passes) ntasks_max += (ps.count + kTaskCandidates[0] - 1) / kTaskCandidates[0];
   b->nobs = (uint32_t)obs.size();
   HIP_TRY(hipMalloc(&b->d_chains, nc * sizeof(dsa::BatchChain)));
-  HIP_TRY(hipMalloc(&b->d_tasks, std::max<size_t>(1, tasks.size()) * sizeof(dsa::BatchTask)));
+  HIP_TRY(hipMalloc(&b->d_tasks, std::max<size_t>(1, ntasks_max) * sizeof(dsa::BatchTask)));
   HIP_TRY(hipMalloc(&b->d_obs, obs.size() * sizeof(dsa::ObsTask)));
   HIP_TRY(hipMalloc(&b->d_obs_first, nc * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&b->d_obs_n, nc * sizeof(uint32_t)));
   HIP_TRY(hipMalloc(&b->d_partials, 4 * obs.size() * sizeof(double)));
   HIP_TRY(hipMalloc(&b->d_out, 4 * nc * sizeof(double)));
   HIP_TRY(hipMemcpy(b->d_chains, desc.data(), nc * sizeof(dsa::BatchChain), hipMemcpyHostToDevice));
-  if (!tasks.empty())
-    HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(dsa::BatchTask), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(b->d_obs, obs.data(), obs.size() * sizeof(dsa::ObsTask), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(b->d_obs_first, ofirst.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(b->d_obs_n, on.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (int rc = b->task_links ? set_tasks(b, b->task_links) : tune_tasks(b, nlinks, nms)) return rc;
   b->final = true;
+  return 0;
+}
+
+int dsa_batch_set_task_links(dsa_batch *b, uint32_t links) {
+  if (!b) return fail(DSA_E_ARG, "dsa_batch_set_task_links: NULL batch");
+  if (b->final) return fail(DSA_E_STATE, "dsa_batch_set_task_links after dsa_batch_finalize");
+  bool ok = links == 0;
+  for (uint32_t c : kTaskCandidates) ok = ok || links == c;
+  if (!ok) return fail(DSA_E_ARG, "dsa_batch_set_task_links: links must be 0, 256, 512 or 1024");
+  b->task_links = links;
+  return 0;
+}
+
+int dsa_batch_task_links(dsa_batch *b, uint32_t *links, double *ms_per_sweep, uint32_t ncand) {
+  if (!b || !links) return fail(DSA_E_ARG, "dsa_batch_task_links: NULL argument");
+  if (!b->final) return fail(DSA_E_STATE, "batch not finalized");
+  *links = b->block;
+  if (ms_per_sweep)
+    for (uint32_t k = 0; k < ncand; ++k) ms_per_sweep[k] = k < b->tune_ms.size() ? b->tune_ms[k] : 0.0;
   return 0;
 }
 

@@ -197,7 +197,7 @@ int main_llr(int argc, char **argv) {
   std::vector<int> reached(2 * R, 0);
   double maxdrift = 0;
   {
-    Runner run(argv[0], g, D, 2 * R);
+    Runner run(argv[0], g, D, std::max<uint32_t>(2 * R, 1));
     if (!run.ok()) return die(argv[0], "dsa_llr_create");
     // one ladder per (repeat, direction): hot walking down, cold walking up
     std::function<void(unsigned, int, int, double, std::vector<int32_t>)> window;

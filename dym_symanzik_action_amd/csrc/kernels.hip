@@ -102,12 +102,12 @@ void k_sweep_cls(SweepArgs A) {
   }
 }
 
-// Many chains, one pass index: block b serves task[b] = up to kBlock links of
-// one chain's pass, with that chain's tables in LDS.  The sweep number of
+// Many chains, one pass index: block b serves task[b] = up to blockDim.x
+// links of one chain's pass (one per lane), with that chain's tables in LDS.  The sweep number of
 // chain c is its base plus the batch counter kept in device memory (so a
 // captured graph can be replayed unchanged).
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_sweep_batch(const BatchChain *chains, const BatchTask *tasks,
+__global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain *chains, const BatchTask *tasks,
                                                         const unsigned long long *sweep_ctr,
                                                         unsigned long long *acc_out) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -479,10 +479,11 @@ hipError_t launch_kentucky(double *out, const uint16_t *links, const double *mat
 }
 
 hipError_t launch_sweep_batch(int D, const BatchChain *chains, const BatchTask *tasks, uint32_t ntasks,
-                              size_t lds_bytes, const unsigned long long *sweep_ctr, unsigned long long *acc,
-                              hipStream_t s) {
+                              uint32_t block, size_t lds_bytes, const unsigned long long *sweep_ctr,
+                              unsigned long long *acc, hipStream_t s) {
   if (!ntasks) return hipSuccess;
-  const dim3 g(ntasks), b(kBlock);
+  if (block == 0 || block > (uint32_t)kBatchBlockMax || block % 64) return hipErrorInvalidValue;
+  const dim3 g(ntasks), b(block);
   switch (D) {
     case 2: hipLaunchKernelGGL(k_sweep_batch<2>, g, b, lds_bytes, s, chains, tasks, sweep_ctr, acc); break;
     case 3: hipLaunchKernelGGL(k_sweep_batch<3>, g, b, lds_bytes, s, chains, tasks, sweep_ctr, acc); break;

@@ -165,8 +165,12 @@ hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n
 hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, uint32_t n, const ObsTask *tasks,
                              uint32_t ntasks, hipStream_t s);
 
+// `block` lanes per block (a multiple of 64, at most kBatchBlockMax); every
+// task holds at most `block` links.
+constexpr int kBatchBlockMax = 1024;
 hipError_t launch_sweep_batch(int D, const BatchChain *chains, const BatchTask *tasks, uint32_t ntasks,
-                              size_t lds_bytes, const unsigned long long *sweep_ctr, unsigned long long *acc,
+                              uint32_t block, size_t lds_bytes, const unsigned long long *sweep_ctr,
+                              unsigned long long *acc,
                               hipStream_t s);
 hipError_t launch_count_sweep(unsigned long long *sweep_ctr, hipStream_t s);
 // out[c*4 + q]: mean Re P, Im P, plaquette, 2x1 rectangle of chain c; partials

@@ -88,6 +88,8 @@ _SIGS = {
     "dsa_batch_finalize": (C.c_int, [C.c_void_p]),
     "dsa_batch_set_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
     "dsa_batch_exact_decisions": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "dsa_batch_set_task_links": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "dsa_batch_task_links": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p, C.c_uint32]),
     "dsa_batch_destroy": (None, [C.c_void_p]),
     "dsa_batch_sweep": (C.c_int, [C.c_void_p, C.c_uint32]),
     "dsa_batch_measure": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -387,6 +389,17 @@ class Batch:
         n = C.c_uint64()
         _check(lib().dsa_batch_exact_decisions(self._h, C.byref(n)))
         return n.value
+
+    def set_task_links(self, links: int = 0) -> None:
+        """Before finalize: links per sweep block (256, 512, 1024) or 0 to tune."""
+        _check(lib().dsa_batch_set_task_links(self._h, links))
+
+    def task_links(self):
+        """(links per block in use, ms per sweep of 256/512/1024 when tuned)."""
+        n = C.c_uint32()
+        ms = np.zeros(3, np.float64)
+        _check(lib().dsa_batch_task_links(self._h, C.byref(n), _ptr(ms), 3))
+        return n.value, ms
 
     def finalize(self) -> "Batch":
         _check(lib().dsa_batch_finalize(self._h))

@@ -176,6 +176,15 @@ int dsa_batch_finalize(dsa_batch *b);
 int dsa_batch_set_kernel(dsa_batch *b, int32_t kernel, double margin_scale);
 /* Hits of the batch's class updates re-decided from element values. */
 int dsa_batch_exact_decisions(dsa_batch *b, uint64_t *n);
+/* Before finalize: links per sweep block (one per lane): 256, 512 or 1024,
+ * or 0 (default) to let finalize time each size over two sweeps and keep the
+ * fastest, restoring the chains' state afterwards, so results do not depend
+ * on the choice (no counterpart in the reference, which runs one process per
+ * point). */
+int dsa_batch_set_task_links(dsa_batch *b, uint32_t links);
+/* The size in use and, when tuned, ms per sweep of each candidate (256, 512,
+ * 1024; the first ncand written to ms_per_sweep, which may be NULL). */
+int dsa_batch_task_links(dsa_batch *b, uint32_t *links, double *ms_per_sweep, uint32_t ncand);
 void dsa_batch_destroy(dsa_batch *b);
 /* n sweeps of every chain (update(), dym-mod-metro.cpp:146-197). */
 int dsa_batch_sweep(dsa_batch *b, uint32_t n);

@@ -61,6 +61,31 @@ def test_batch_equals_lone_chains(dsa, dgroups, kernel):
             assert np.allclose(got[c], m[:4], rtol=0, atol=1e-13 * 3)
 
 
+def test_block_size_does_not_change_results(dsa, dgroups):
+    """Every fixed block size (256, 512, 1024 links) gives the tuned batch's
+    links and counters: the tuner restores the state it sweeps."""
+    say("case batch block sizes")
+    with build(dsa, dgroups, MIX) as ref:
+        ref.sweep(3)
+        links, ms = ref.task_links()
+        assert links in (256, 512, 1024) and (ms > 0).all()
+        want = [ref.links(c) for c in range(len(MIX))]
+        counters = ref.counters()
+    for n in (256, 512, 1024):
+        b = dsa.Batch()
+        b.set_task_links(n)
+        for name, D, Nt, Nx, b0, b1, b2, seed, start, rng in MIX:
+            b.add(dgroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng)
+        with b.finalize() as b:
+            assert b.task_links()[0] == n
+            b.sweep(3)
+            for c in range(len(MIX)):
+                assert np.array_equal(b.links(c), want[c]), (n, c)
+            assert all(np.array_equal(x, y) for x, y in zip(b.counters(), counters))
+    with pytest.raises(dsa.DSAError):
+        dsa.Batch().set_task_links(300)
+
+
 def test_graph_cycles_replay_like_eager(dsa, dgroups):
     say("case graph cycles")
     mix = MIX[:3] + MIX[5:6]
