@@ -119,6 +119,7 @@ int dsa::check_params(const dsa_chain_params *p) {
   if (p->D < 2 || p->D > (uint32_t)dsa::kDMax)
     return fail(DSA_E_RANGE, "D = " + std::to_string(p->D) + " outside the supported 2..5");
   if (p->Nt < 1 || p->Nx < 1) return fail(DSA_E_ARG, "Nt and Nx must be positive");
+  if (p->Nt > 4095 || p->Nx > 4095) return fail(DSA_E_RANGE, "Nt and Nx must be below 4096");
   uint64_t V = p->Nt;
   for (uint32_t d = 1; d < p->D; ++d) V *= p->Nx;
   if (V * p->D >= (1ull << 31)) return fail(DSA_E_RANGE, "V*D must be below 2^31 links");

@@ -28,9 +28,13 @@ __device__ inline int wrapc(int v, int n) {
   return v;
 }
 
+// A site: its index, parity and coordinates packed 12 bits each (extents
+// below 4096), so that a coordinate along a run-time direction is a shift,
+// not an index into a register array (which the compiler would place in
+// scratch memory).
 template <int D>
 struct Site {
-  int x[D];
+  uint64_t xs;
   uint32_t i;
   uint32_t par;
 };
@@ -39,25 +43,23 @@ template <int D>
 __device__ inline Site<D> decode(const Geom &G, uint32_t i) {
   Site<D> s;
   s.i = i;
+  s.xs = 0;
   uint32_t par = 0;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const uint32_t q = i / G.ext[d];
-    s.x[d] = (int)(i - q * G.ext[d]);
-    par += (uint32_t)s.x[d];
+    const uint32_t x = i - q * G.ext[d];
+    s.xs |= (uint64_t)x << (12 * d);
+    par += x;
     i = q;
   }
   s.par = par & 1u;
   return s;
 }
 
-// s.x[d] for a run-time d without indexing a register array (no scratch).
 template <int D>
 __device__ inline int coord(const Site<D> &s, int d) {
-  int r = s.x[0];
-#pragma unroll
-  for (int q = 1; q < D; ++q) r = (q == d) ? s.x[q] : r;
-  return r;
+  return (int)((s.xs >> (12 * d)) & 0xfffu);
 }
 
 // Site at x + a e_mu + b e_nu (periodic); its parity in *par.

@@ -105,12 +105,26 @@ __device__ inline double llr_margin(const LlrChain &LC, double A1, double A2) {
   return LC.cmscale * (2.0 * vals * (1.0 + 1e-6) + 8.0 * rounding);
 }
 
-// Certified comparison exp(-x) > u from x_c with |x - x_c| <= M (dsa_exp within
-// one ulp on both sides): 1 true, 0 false, -1 undecided.
+// Certified comparison exp(-x) > u (u in [0, 1)) from x_c with |x - x_c| <= M:
+// 1 true, 0 false, -1 undecided.  Below the band exp(-x) >= 1 > u.  Above
+// it, v_exp_f32 (2^y, ~1 ulp) on the float-rounded argument is within 5e-6
+// relative of exp(-x_c) for x_c < 60, so it decides with 1.5e-5 of slack,
+// exp(-x) < exp(-37) < 1e-15 <= u is false, and what is left is decided
+// with dsa_exp (within one ulp).  Inside the band exp(-x) >= 1 - 2M.
 __device__ inline int exp_gt_certified(double xc, double M, double u) {
-  const double e = dsa_exp(-xc), w = 1.02 * M + 0x1p-46;
-  if (w >= 0.5) return -1;
-  return u < e * (1.0 - w) ? 1 : (u >= e * (1.0 + w) ? 0 : -1);
+  if (xc + M <= 0.0) return 1;
+  if (xc - M > 0.0) {
+    if (xc - M > 37.0 && u >= 1e-15) return 0;
+    if (xc < 60.0) {
+      const double e = (double)__builtin_amdgcn_exp2f((float)(-xc * 1.4426950408889634)), w = 1.02 * M + 0x1p-16;
+      if (u < e * (1.0 - w)) return 1;
+      if (u >= e * (1.0 + w)) return 0;
+    }
+    const double ed = dsa_exp(-xc), wd = 1.02 * M + 0x1p-46;
+    if (wd >= 0.5) return -1;
+    return u < ed * (1.0 - wd) ? 1 : (u >= ed * (1.0 + wd) ? 0 : -1);
+  }
+  return u < 1.0 - (2.02 * M + 0x1p-46) ? 1 : -1;
 }
 
 // Certified sign test y > 0 from y_c with |y - y_c| <= M.
@@ -174,33 +188,17 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     const uint32_t i = BC.sites[idx];
     const int mu = tk.mu;
     const Site<D> s = decode<D>(G, i);
-    uint32_t stp[NS], re[NR];
-    build_staples<D>(BC.links, G, T, s, mu, stp, re);
     const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
     uint32_t b = BC.links[lp];
     const uint32_t b0 = b;
-    // element sums (reference order) in double and in fixed point
-    double r1 = 0, m1 = 0;
-    long long q1 = 0, q2 = 0;
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-      const double v = s_retr[T.M(b, stp[j])];
-      r1 += v;
-      q1 += to_fix(v);
-    }
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      const double v = s_retr[T.M(b, re[q])];
-      m1 += v;
-      q2 += to_fix(v);
-    }
     const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
     uint32_t nacc = 0;
+    long long d1 = 0, d2 = 0;
     if (cls) {
-      // t_k = s_k b: ReTr(b p s_k) = ReTr(t_k p), of the class of pool[j] t_k
+      // t_k = s_k b, built directly: ReTr(b p s_k) = ReTr(t_k p), of the class
+      // of pool[j] t_k; the element value ReTr(b s_k) is retr[b t_k b^-1]
       uint32_t tt[NL];
-#pragma unroll
-      for (int k = 0; k < NL; ++k) tt[k] = T.M(k < NS ? stp[k] : re[k - NS], b);
+      build_t<D>(BC.links, G, T, s, mu, b, tt);
       auto sums = [&](ClsNarrow::Row row, int32_t &rc, int32_t &mc) {
         int32_t x = 0, y = 0;
 #pragma unroll
@@ -223,18 +221,19 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
         const double M = margin + 0x1p-50 * fabs(xc);
         // drive-in: axes 1, 2 accept sign dEw > 0; axis 3 accepts gx dS1 + gy dS2 < 0
         int dec = drive ? (axis == 3 ? gt0_certified(-xc, M) : gt0_certified(xc, M)) : exp_gt_certified(xc, M, u2);
-        if (dec < 0) {  // decide exactly as the element path does
-          const uint32_t bnew = T.M(b, s_pool[jp]);
+        if (dec < 0) {  // decide exactly as the element path does, from the same sums
+          const uint32_t bnew = T.M(b, s_pool[jp]), bi = T.I(b);
           double e1 = 0, e2 = 0, e1n = 0, e2n = 0;
 #pragma unroll
-          for (int j = 0; j < NS; ++j) {
-            e1 += s_retr[T.M(b, stp[j])];
-            e1n += s_retr[T.M(bnew, stp[j])];
-          }
-#pragma unroll
-          for (int q = 0; q < NR; ++q) {
-            e2 += s_retr[T.M(b, re[q])];
-            e2n += s_retr[T.M(bnew, re[q])];
+          for (int k = 0; k < NL; ++k) {
+            const double v = s_retr[T.M(T.M(b, tt[k]), bi)], vn = s_retr[T.M(T.M(bnew, tt[k]), bi)];
+            if (k < NS) {
+              e1 += v;
+              e1n += vn;
+            } else {
+              e2 += v;
+              e2n += vn;
+            }
           }
           const double dS1 = e1n - e1, dS2 = e2n - e2;
           bool accept;
@@ -256,7 +255,43 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
           ++nacc;
         }
       }
+      if (b != b0) {  // energy change in fixed point: s_k = t_k b^-1
+        BC.links[lp] = (uint16_t)b;
+        const uint32_t bi = T.I(b);
+        long long p1 = 0, p2 = 0, q1 = 0, q2 = 0;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+          const uint32_t sk = T.M(tt[k], bi);
+          const long long vp = to_fix(s_retr[T.M(b, sk)]), vq = to_fix(s_retr[T.M(b0, sk)]);
+          if (k < NS) {
+            p1 += vp;
+            q1 += vq;
+          } else {
+            p2 += vp;
+            q2 += vq;
+          }
+        }
+        d1 = p1 - q1;
+        d2 = p2 - q2;
+      }
     } else {
+      uint32_t stp[NS], re[NR];
+      build_staples<D>(BC.links, G, T, s, mu, stp, re);
+      // element sums (reference order) in double and in fixed point
+      double r1 = 0, m1 = 0;
+      long long q1 = 0, q2 = 0;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const double v = s_retr[T.M(b, stp[j])];
+        r1 += v;
+        q1 += to_fix(v);
+      }
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const double v = s_retr[T.M(b, re[q])];
+        m1 += v;
+        q2 += to_fix(v);
+      }
       for (int h = 0; h < kNHit; ++h) {
         const U4 w = philox4x32_10(U4{link, (uint32_t)h, sw_lo, sw_hi}, BC.seed, kKeyLLR);
         const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
@@ -288,17 +323,16 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
           ++nacc;
         }
       }
-    }
-    long long d1 = 0, d2 = 0;
-    if (b != b0) {
-      BC.links[lp] = (uint16_t)b;
-      long long p1 = 0, p2 = 0;
+      if (b != b0) {
+        BC.links[lp] = (uint16_t)b;
+        long long p1 = 0, p2 = 0;
 #pragma unroll
-      for (int j = 0; j < NS; ++j) p1 += to_fix(s_retr[T.M(b, stp[j])]);
+        for (int j = 0; j < NS; ++j) p1 += to_fix(s_retr[T.M(b, stp[j])]);
 #pragma unroll
-      for (int q = 0; q < NR; ++q) p2 += to_fix(s_retr[T.M(b, re[q])]);
-      d1 = p1 - q1;
-      d2 = p2 - q2;
+        for (int q = 0; q < NR; ++q) p2 += to_fix(s_retr[T.M(b, re[q])]);
+        d1 = p1 - q1;
+        d2 = p2 - q2;
+      }
     }
     LC.old[idx] = (uint16_t)b0;
     LC.dE1[idx] = d1;
