@@ -274,3 +274,60 @@ def test_llr2d_solve_matches_oracle_on_reference_cells(oracle, ogroups, tmp_path
         st = finish(o)
         want.append(cfgio.ane2_line(c["E1"], c["E2"], st["a"], st["a2"]))
     assert rows == want
+
+
+def test_llr_axis2_two_repeats_match_oracle_replay(oracle, ogroups):
+    """Axis 2 (the rectangle action windowed, bg = beta1) on 4^4 with two
+    repeats of both ladders (R = 2): each of the four ladders replayed by the
+    oracle, and the ANE lines' means and deviations over the four."""
+    args = ["4", "4", "4", "2", "1.0", "-1200", "-1400", "100", "80", "0.5", "9", "3", "2", "2"]
+    say("case llr cli axis 2, R = 2", args)
+    out, err = run(LLR, [group_file("S216")] + args)
+    T = tasks(err)
+    D, Nt, Nx, axis = 4, 4, 4, 2
+    bg, Etop, Ebot, step, hw, a0 = 1.0, -1200.0, -1400.0, 100.0, 80.0, 0.5
+    K, NRM, R = 3, 2, 2
+    M = int((Etop - Ebot) / step + 0.5) + 1
+    an = {n: [] for n in range(M)}
+    for r in range(R):
+        for d in ("down", "up"):
+            links, a_prev = None, a0
+            for s in range(M):
+                n = s if d == "down" else M - 1 - s
+                t = T[f"{r} {d} {n}"]
+                o = oracle.OLlr(ogroups("S216"), D, Nt, Nx, axis, bg, Etop - n * step, hw, a_prev, t["seed"],
+                                start="hot" if d == "down" else "cold", K=K, NRM=NRM, cap=20000, links=links)
+                st = finish(o)
+                assert (st["reached"], st["a"]) == (t["reached"], t["a"]), (r, d, n)
+                if not st["reached"]:
+                    break
+                an[n].append(st["a"])
+                links, a_prev = o.a_links, st["a"]
+    lines = out.splitlines()
+    assert sum(l.startswith("repeat ") for l in lines) == R
+    got = {float(l.split()[1]): l.split() for l in lines if l.startswith("ANE:") and "unreached" not in l}
+    for n in range(M):
+        if not an[n]:
+            continue
+        row = got[Etop - n * step]
+        m, sd = mean_sd(an[n])
+        # the drop-in collects the ladders as they finish: the same values in
+        # another order, so the mean may differ in the last bit of %9.5f only
+        assert abs(float(row[2]) - m) <= 1.01e-5 and abs(float(row[3]) - sd) <= 1.01e-5 and int(row[4]) == len(an[n])
+
+
+def test_llr2d_solve_skips_missing_cells(tmp_path):
+    """A manifest line whose configuration file is missing is reported on
+    stderr and skipped (llr2d.cpp:500-501); the other cells are solved."""
+    f = np.load(os.path.join(GOLDEN, "llr2d_seed_3x3.npz"))
+    (tmp_path / "cfg").mkdir()
+    man = [str(l) for l in f["manifest"]][:3]
+    for l in man[1:]:
+        p = l.rsplit(" ", 1)[1]
+        f[os.path.basename(p)[:-4]].tofile(tmp_path / p)
+    (tmp_path / "m.txt").write_text("\n".join(man) + "\n")
+    out, err = run(LLR2D, ["solve", group_file("mys1080-v4"), "4", "3", "3", "m.txt", "0", "3", "1.0", "2", "2", "5"],
+                   cwd=str(tmp_path), trace=False)
+    assert "solve: missing cfg cfg/" in err and "SOLVE[0,3): 2 cells" in err
+    assert [l.split()[1:3] for l in out.splitlines() if l.startswith("ANE2:")] == \
+        [l.split()[4:6] for l in man[1:]]
