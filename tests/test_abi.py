@@ -128,7 +128,7 @@ def test_chain_without_device_fails_cleanly(dsa, dgroups):
 
 
 def test_chain_rejects_bad_shapes(dsa, dgroups):
-    for args in [(1, 4, 4), (6, 4, 4), (4, 0, 4)]:
+    for args in [(1, 4, 4), (6, 4, 4), (4, 0, 4), (4, 4096, 4), (2, 8, 5000)]:
         with pytest.raises(dsa.DSAError) as e:
             dsa.Chain(dgroups("S108"), *args, 0, 1, 0, 1)
         assert e.value.code in (-1, -5)
