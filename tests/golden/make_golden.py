@@ -6,7 +6,7 @@ It runs oracle/_ref/dym-mod-metro (the reference driver built from its own
 sources) and oracle/_ref/ref_harness (the reference's own get_staples,
 get_rect_loops, update, plaquette, polyakov, wilson, getpoly called directly)
 and stores their inputs and outputs as small .npz / .txt files.  Nothing here
-is run on the GPU box: the tests only read the stored files.
+runs where the GPU tests run: the tests only read the stored files.
 """
 import json
 import os
