@@ -154,6 +154,9 @@ __host__ __device__ inline size_t llr_lds_bytes(const Tables &T, bool cls) {
   return cls ? pad16(e) + pad16(T.P) + pad16((size_t)T.npool * T.P) : e;
 }
 
+// Static LDS of the LLR pass kernel (class values, the class path's uniforms).
+size_t llr_pass_static_lds();
+
 // Proposals: every link of pass `pass` of every active chain.
 hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const BatchTask *tasks, uint32_t ntasks,
                            size_t lds_bytes, hipStream_t s);

@@ -134,6 +134,11 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, LlrState *st, const BatchTask *tasks) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t s_rcq[64];  // class values (2^-24), indexed by the byte tables
+  // class path: the hits' proposal indices and acceptance uniforms, drawn by
+  // the whole block before the links are updated (a block of fewer links than
+  // lanes draws them several lanes per link)
+  __shared__ double s_u2[kNHit][kBlock];
+  __shared__ uint16_t s_jp[kNHit][kBlock];
   const BatchTask tk = tasks[blockIdx.x];
   const LlrChain &LC = chains[tk.chain];
   const LlrState &S = st[tk.chain];
@@ -155,6 +160,15 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     const uint4 *src = reinterpret_cast<const uint4 *>(BC.T.cprod);
     uint4 *dst = reinterpret_cast<uint4 *>(s_cprod);
     for (uint32_t t = threadIdx.x; t < n16; t += blockDim.x) dst[t] = src[t];
+    const uint32_t sl = (uint32_t)S.sweep, sh = (uint32_t)(S.sweep >> 32), n = tk.count;
+    const double npd = (double)np;
+    for (uint32_t j = threadIdx.x; j < n * (uint32_t)kNHit; j += blockDim.x) {
+      const uint32_t l = j % n, h = j / n;
+      const uint32_t lk = BC.sites[tk.offset + l] * (uint32_t)D + (uint32_t)tk.mu;
+      const U4 w = philox4x32_10(U4{lk, h, sl, sh}, BC.seed, kKeyLLR);
+      s_jp[h][l] = (uint16_t)(npd * u53(w.x, w.y));
+      s_u2[h][l] = u53(w.z, w.w);
+    }
   }
   __syncthreads();
 
@@ -212,9 +226,8 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
       int32_t r1c, m1c;
       sums(Lk.row0(), r1c, m1c);
       for (int h = 0; h < kNHit; ++h) {
-        const U4 w = philox4x32_10(U4{link, (uint32_t)h, sw_lo, sw_hi}, BC.seed, kKeyLLR);
-        const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
-        const uint32_t jp = (uint32_t)(npool_d * u1);
+        const double u2 = s_u2[h][t];
+        const uint32_t jp = s_jp[h][t];
         int32_t r1nc, m1nc;
         sums(Lk.row(jp, P), r1nc, m1nc);
         const double xc = A1q * ((double)r1nc - (double)r1c) + A2q * ((double)m1nc - (double)m1c);
@@ -626,6 +639,8 @@ __global__ void k_llr_resync_apply(const LlrChain *chains, LlrState *st, uint32_
 inline dim3 lanes(uint32_t n) { return dim3((n + 63) / 64); }
 
 }  // namespace
+
+size_t llr_pass_static_lds() { return kNHit * kBlock * (sizeof(double) + sizeof(uint16_t)) + 64 * sizeof(int32_t); }
 
 hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const BatchTask *tasks, uint32_t ntasks,
                            size_t lds_bytes, hipStream_t s) {

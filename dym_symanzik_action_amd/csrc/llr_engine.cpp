@@ -22,6 +22,8 @@ using dsa::fail;
 namespace {
 
 constexpr uint32_t kTaskLinks = 256;
+constexpr uint32_t kTaskLinksSmall = 64;  // for passes below kSmallPass links in all
+constexpr uint64_t kSmallPass = 32768;
 constexpr uint32_t kObsSites = 1024;
 
 struct Slot {
@@ -84,12 +86,18 @@ int rebuild_tasks(dsa_llr *L) {
   L->pass_n.clear();
   for (size_t q = 0; q < maxpass; ++q) {
     L->pass_off.push_back((uint32_t)tasks.size());
+    // a pass that would not fill the device is cut into blocks of fewer
+    // links, so that more lanes share each link's random draws
+    uint64_t total = 0;
+    for (const Slot &s : L->slots)
+      if (s.used && q < s.S.passes.size()) total += s.S.passes[q].count;
+    const uint32_t per = total < kSmallPass ? kTaskLinksSmall : kTaskLinks;
     for (uint32_t c = 0; c < L->slots.size(); ++c) {
       const Slot &s = L->slots[c];
       if (!s.used || q >= s.S.passes.size()) continue;
       const dsa::Pass &ps = s.S.passes[q];
-      for (uint32_t o = 0; o < ps.count; o += kTaskLinks)
-        tasks.push_back(dsa::BatchTask{c, ps.offset + o, std::min(kTaskLinks, ps.count - o), ps.mu});
+      for (uint32_t o = 0; o < ps.count; o += per)
+        tasks.push_back(dsa::BatchTask{c, ps.offset + o, std::min(per, ps.count - o), ps.mu});
     }
     L->pass_n.push_back((uint32_t)tasks.size() - L->pass_off.back());
   }
@@ -236,8 +244,9 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   const dsa::Tables T = dsa::tables_view(g, s.dt);
   if (dsa::llr_lds_bytes(T, false) > 65536) return fail(DSA_E_RANGE, "group order too large for the LDS tables");
   // the class path where the byte class tables exist and fit beside the
-  // static class values (256 B)
-  const bool cls = L->kernel != DSA_KERNEL_TABLE && T.cprod && dsa::llr_lds_bytes(T, true) + 256 <= L->lds_max;
+  // kernel's static arrays
+  const bool cls = L->kernel != DSA_KERNEL_TABLE && T.cprod &&
+                   dsa::llr_lds_bytes(T, true) + dsa::llr_pass_static_lds() <= L->lds_max;
   if ((L->kernel == DSA_KERNEL_CLASS || L->kernel == DSA_KERNEL_CLASS_NARROW) && !cls)
     return fail(DSA_E_STATE, "no class tables for this group on this device");
   std::string err = dsa::build_colour_schedule(s.G, s.S);
