@@ -427,7 +427,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.beta2 = c->p.beta2;
   A.seed = (uint32_t)c->p.seed;
   A.fall = c->d_fall;
-  // the class form this chain runs for passes large enough to fill the device
+  // the class form this chain runs (kClsNone: the table kernel)
   const int mode = cls_mode(c);
   A.cmargin = (mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
   const size_t np = c->S.passes.size();
@@ -447,7 +447,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
       A.mu = ps.mu;
       A.sites = c->d_sites + ps.offset;
       A.count = ps.count;
-      A.use_cls = (c->kernel != DSA_KERNEL_AUTO || ps.count >= 2048) ? mode : dsa::kClsNone;
+      A.use_cls = mode;
       if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q], c->stream));
       HIP_TRY(dsa::launch_sweep(A, c->max_blocks, c->stream));
       if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q + 1], c->stream));

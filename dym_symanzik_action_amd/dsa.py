@@ -271,8 +271,8 @@ class Chain:
         _check(lib().dsa_chain_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
 
     def kernel_form(self) -> str:
-        """The kernel passes of at least 2048 links run: "table", "class" (the
-        int16-value form) or "class-narrow"."""
+        """The kernel the chain's passes run: "table", "class" (the int16-value
+        form) or "class-narrow"."""
         f = C.c_int32()
         _check(lib().dsa_chain_kernel_form(self._h, C.byref(f)))
         return {v: k for k, v in DSA_KERNEL.items()}[f.value]

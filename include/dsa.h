@@ -142,15 +142,16 @@ int dsa_chain_save_kentucky(dsa_chain *c, const char *prefix, int32_t num, char 
  * else byte class indices); a hit whose decision could depend on the
  * quantisation or the table's within-class ReTr rounding is decided again
  * from element values.  CLASS_NARROW forces the byte-index form.  AUTO
- * (default): the class kernel for passes of at least 2048 links when the
- * group's class tables fit the device's LDS, in the byte-index form where
- * that fits, else as CLASS.  margin_scale (default 1) multiplies the bound
+ * (default): the class kernel when the group's class tables fit the
+ * device's LDS (it is faster at every lattice size measured, 8^4 to
+ * 32^3 x 64), in the byte-index form where that fits, else as CLASS.
+ * margin_scale (default 1) multiplies the bound
  * that selects those re-decisions: a huge value makes every hit re-decided
  * (a test of that path). */
 enum { DSA_KERNEL_AUTO = 0, DSA_KERNEL_TABLE = 1, DSA_KERNEL_CLASS = 2, DSA_KERNEL_CLASS_NARROW = 3 };
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale);
-/* The kernel the chain's passes of at least 2048 links run: TABLE, CLASS
- * (the int16-value form) or CLASS_NARROW (the byte-index form). */
+/* The kernel the chain's passes run: TABLE, CLASS (the int16-value form) or
+ * CLASS_NARROW (the byte-index form). */
 int dsa_chain_kernel_form(dsa_chain *c, int32_t *form);
 /* Hits the CLASS kernel re-decided from element values since creation. */
 int dsa_chain_exact_decisions(dsa_chain *c, uint64_t *n);
