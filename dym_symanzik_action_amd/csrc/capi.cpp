@@ -62,6 +62,7 @@ struct dsa_chain {
   int kernel = DSA_KERNEL_AUTO;
   int cls_best = dsa::kClsNone;
   bool narrow_ok = false;
+  size_t lds_max = 0;  // LDS bytes per block on the chain's device
   double margin8 = 0, margin16 = 0, margin_scale = 1;
   // per-launch timing (measurement mode): one event pair per pass
   bool profiling = false;
@@ -368,8 +369,10 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
     // quantisation: half a unit of the class values (2^-24 narrow, 2^-13 wide)
     c->margin8 = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2, 0x1p-25);
     c->margin16 = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2, 0x1p-14);
-    c->narrow_ok = dsa::sweep_cls_lds_bytes(c->T, dsa::kClsNarrow) <= (size_t)lds_max;
-    const bool wide_ok = c->T.cval && dsa::sweep_cls_lds_bytes(c->T, dsa::kClsWide) <= (size_t)lds_max;
+    c->lds_max = (size_t)lds_max;
+    // + the kernels' static class-value array
+    c->narrow_ok = dsa::sweep_cls_lds_bytes(c->T, dsa::kClsNarrow) + 256 <= (size_t)lds_max;
+    const bool wide_ok = c->T.cval && dsa::sweep_cls_lds_bytes(c->T, dsa::kClsWide) + 256 <= (size_t)lds_max;
     if (wide_ok && c->margin16 < 1e-2)
       c->cls_best = dsa::kClsWide;
     else if (c->narrow_ok && c->margin8 < 1e-3)
@@ -427,6 +430,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.beta2 = c->p.beta2;
   A.seed = (uint32_t)c->p.seed;
   A.fall = c->d_fall;
+  A.lds_max = c->lds_max;
   // the class form this chain runs (kClsNone: the table kernel)
   const int mode = cls_mode(c);
   A.cmargin = (mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;

@@ -334,9 +334,19 @@ struct ClsCtx {
   double margin;
 };
 
+// Uniforms drawn ahead by the block (Philox streams): hit h of the block's
+// link l has proposal index jp[h * stride + l] and acceptance uniform
+// u2[h * stride + l].
+struct PreDraws {
+  const uint16_t *jp;
+  const double *u2;
+  uint32_t stride, l;
+};
+
 template <int D, class L>
 __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const ClsCtx &K, const Geom &G,
-                                           const TabView &T, int mu, uint32_t i, uint32_t &nfall) {
+                                           const TabView &T, int mu, uint32_t i, uint32_t &nfall,
+                                           const PreDraws *pre = nullptr) {
   constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1), NL = NS + NR;
   const Site<D> s = decode<D>(G, i);
   const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
@@ -363,16 +373,23 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
   const double b1q = C.beta1 * L::kScale, b2q = C.beta2 * L::kScale, margin = K.margin;
   uint32_t acc = 0;
   for (int h = 0; h < kNHit; ++h) {
-    double u1, u2;
-    if (C.rng == kRngPhilox) {
-      const U4 w = philox4x32_10(U4{link, (uint32_t)h, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
-      u1 = u53(w.x, w.y);
-      u2 = u53(w.z, w.w);
+    double u2;
+    uint32_t jp;
+    if (pre) {
+      jp = pre->jp[h * pre->stride + pre->l];
+      u2 = pre->u2[h * pre->stride + pre->l];
     } else {
-      u1 = minstd_canonical(ms);
-      u2 = minstd_canonical(ms);
+      double u1;
+      if (C.rng == kRngPhilox) {
+        const U4 w = philox4x32_10(U4{link, (uint32_t)h, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
+        u1 = u53(w.x, w.y);
+        u2 = u53(w.z, w.w);
+      } else {
+        u1 = minstd_canonical(ms);
+        u2 = minstd_canonical(ms);
+      }
+      jp = (uint32_t)(C.npool_d * u1);
     }
-    const uint32_t jp = (uint32_t)(C.npool_d * u1);
     int32_t r1n, m1n;
     sums(Lk.row(jp, T.P), r1n, m1n);
     // class-valued change of action; the exact one lies within M of it

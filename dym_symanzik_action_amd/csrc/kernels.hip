@@ -102,6 +102,67 @@ void k_sweep_cls(SweepArgs A) {
   }
 }
 
+// The narrow form for passes too small to fill the device: a 512-lane block
+// takes NL links; all its lanes first draw the links' Philox uniforms
+// (512 / NL lanes per link), so a link's hits wait only on its lookups.  NL
+// is 128 below kClsSmallPass links a pass and 256 below kClsMidPass
+// (measured on 8^4 ... 32^3x64 lattices); larger passes fill the device
+// with k_sweep_cls, whose 16 waves a CU hide the draws.
+constexpr uint32_t kClsSmallPass = 32768, kClsMidPass = 65536;
+// static LDS of k_sweep_cls_small<D, NL>: class values, then the draws
+constexpr size_t cls_small_static_lds(int nl) {
+  return 64 * sizeof(int32_t) + (size_t)kNHit * nl * (sizeof(double) + sizeof(uint16_t));
+}
+
+template <int D, int NL>
+__global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_rcq[64];
+  __shared__ double s_u2[kNHit * NL];
+  __shared__ uint16_t s_jp[kNHit * NL];
+  const uint32_t P = A.T.P, np = A.T.npool, nc = A.T.ncls;
+  uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem);
+  uint16_t *s_pool = reinterpret_cast<uint16_t *>(smem + pad16(2 * (size_t)P));
+  unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
+  unsigned char *s_tab = s_tab0 + pad16(P);
+  for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+    s_inv[t] = A.T.inv[t];
+    s_tab0[t] = A.T.cls[t];
+  }
+  for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) s_pool[t] = A.T.pool[t];
+  for (uint32_t t = threadIdx.x; t < nc; t += blockDim.x) s_rcq[t] = A.T.rcq[t];
+  copy16(s_tab, A.T.cprod, (size_t)np * P);
+  const uint32_t first = blockIdx.x * NL;
+  const uint32_t n = A.count - first < (uint32_t)NL ? A.count - first : (uint32_t)NL;
+  const double npd = (double)np;
+  for (uint32_t j = threadIdx.x; j < n * (uint32_t)kNHit; j += blockDim.x) {
+    const uint32_t l = j % n, h = j / n;
+    const uint32_t lk = A.sites[first + l] * (uint32_t)D + (uint32_t)A.mu;
+    const U4 w = philox4x32_10(U4{lk, h, A.sweep_lo, A.sweep_hi}, A.seed, kKeySweep);
+    s_jp[h * NL + l] = (uint16_t)(npd * u53(w.x, w.y));
+    s_u2[h * NL + l] = u53(w.z, w.w);
+  }
+  __syncthreads();
+
+  const TabView T{A.T.mult, s_inv, P};
+  const LinkCtx C{A.links, A.mstate, nullptr, s_pool, npd, A.beta0, A.beta1, A.beta2,
+                  A.seed, A.sweep_lo, A.sweep_hi, A.rng};
+  const ClsCtx K{A.T.retr, A.cmargin};
+  const ClsNarrow Lk{s_tab, s_tab0, s_rcq};
+  unsigned long long acc = 0;
+  uint32_t nfall = 0;
+  if (threadIdx.x < n) {
+    const PreDraws pre{s_jp, s_u2, (uint32_t)NL, threadIdx.x};
+    acc = update_link_cls<D>(C, Lk, K, A.G, T, A.mu, A.sites[first + threadIdx.x], nfall, &pre);
+  }
+  acc = wave_sum_u64(acc);
+  const unsigned long long nf = wave_sum_u64(nfall);
+  if ((threadIdx.x & 63u) == 0) {
+    if (acc) atomicAdd(A.acc, acc);
+    if (nf) atomicAdd(A.fall, nf);
+  }
+}
+
 // Many chains, one pass index: block b serves task[b] = up to blockDim.x
 // links of one chain's pass (one per lane), with that chain's tables in LDS.  The sweep number of
 // chain c is its base plus the batch counter kept in device memory (so a
@@ -436,6 +497,31 @@ size_t sweep_cls_lds_bytes(const Tables &T, int mode) {
 
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
   if (A.count == 0) return hipSuccess;
+  const size_t lds_small = sweep_cls_lds_bytes(A.T, kClsNarrow);
+  const bool sm = A.count < kClsSmallPass ||
+                  lds_small + cls_small_static_lds(256) > A.lds_max;  // 128 links where 256 do not fit
+  if (A.use_cls == kClsNarrow && A.rng == kRngPhilox && A.count < kClsMidPass &&
+      lds_small + cls_small_static_lds(sm ? 128 : 256) <= A.lds_max) {
+    if (!A.fall || !A.T.cprod) return hipErrorInvalidValue;
+    const uint32_t nl = sm ? 128 : 256, nb = (A.count + nl - 1) / nl;
+    const size_t lds = lds_small;
+#define DSA_SMALL(d)                                                                                       \
+  case d:                                                                                                \
+    if (sm)                                                                                              \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 128>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);       \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 256>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);       \
+    break;
+    switch (A.G.D) {
+      DSA_SMALL(2)
+      DSA_SMALL(3)
+      DSA_SMALL(4)
+      DSA_SMALL(5)
+      default: return hipErrorInvalidValue;
+    }
+#undef DSA_SMALL
+    return hipGetLastError();
+  }
   if (A.use_cls == kClsNarrow || A.use_cls == kClsWide) {
     const bool w = A.use_cls == kClsWide;
     if (!A.fall || (w ? !A.T.cval : !A.T.cprod)) return hipErrorInvalidValue;

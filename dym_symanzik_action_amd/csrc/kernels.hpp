@@ -57,6 +57,9 @@ struct SweepArgs {
   int use_cls;
   double cmargin;
   unsigned long long *fall;
+  // the device's LDS bytes per block: the small-pass class forms run only
+  // where their draw arrays fit beside the tables
+  size_t lds_max;
 };
 
 size_t sweep_lds_bytes(const Tables &T);

@@ -10,7 +10,9 @@
 * statistics of the default mode: within 3.5 sigma (Gamma-method errors) of
   the reference binary's runs (cli_stats_*.txt); the four stream x order
   combinations sample one ensemble;
-* at the 32^3 x 64 benchmark size: size-independent properties.
+* at the 32^3 x 64 benchmark size: size-independent properties; at pass
+  sizes between the small-pass and device-filling class launches, the class
+  kernel equals the table kernel bit for bit.
 """
 import glob
 import os
@@ -164,6 +166,23 @@ def test_config2_shape_one_sweep_matches_oracle(dsa, dgroups, oracle, ogroups):
         assert np.all(np.abs(m - exact) <= 1e-13 * scale)
         seq = o.measure(8, 8, mode=0)
         assert np.all(np.abs(seq - exact) <= 1e-11 * scale)
+
+
+@pytest.mark.parametrize("Nt,Nx", [(24, 24), (32, 24)])
+def test_mid_size_passes_class_equals_table(dsa, dgroups, Nt, Nx):
+    """Passes of 55,296 and 73,728 links (24^4, 24^3 x 32), between the
+    small-pass and the device-filling class launches: the class kernel's
+    trajectory is the table kernel's, bit for bit, over two sweeps."""
+    say("case class vs table", Nt, Nx)
+    g = dgroups("mys1080-v4")
+    out = []
+    for kernel in ("table", "class-narrow"):
+        with dsa.Chain(g, 4, Nt, Nx, 0.0, 3.0, -0.25, 21) as c:
+            c.set_kernel(kernel)
+            c.update(2)
+            out.append((c.a, c.counters()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
 
 
 def test_resume_from_checkpoint_is_bit_exact(dsa, dgroups):
