@@ -450,13 +450,71 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
   return acc;
 }
 
+// Block-wide staging of a table into LDS in two halves: load() issues a
+// lane's global loads for its first K elements (t = lane + j * blockDim),
+// store() writes them and copies any elements beyond K * blockDim with a
+// plain loop.  Calling every table's load() before any store() puts a whole
+// block's staging behind one round trip to L2; a load-store loop waits on
+// each load before the store of the same iteration (one round trip per
+// iteration, ~18 for the S(1080) class tables at 512 lanes).
+template <int K, typename T>
+struct Staged {
+  T r[K];
+  __device__ inline void load(const T *__restrict__ src, uint32_t n) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const uint32_t t = threadIdx.x + (uint32_t)j * blockDim.x;
+      if (t < n) r[j] = src[t];
+    }
+  }
+  __device__ inline void store(T *dst, const T *__restrict__ src, uint32_t n) const {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const uint32_t t = threadIdx.x + (uint32_t)j * blockDim.x;
+      if (t < n) dst[t] = r[j];
+    }
+    for (uint32_t t = threadIdx.x + (uint32_t)K * blockDim.x; t < n; t += blockDim.x) dst[t] = src[t];
+  }
+};
+
+// The class tables of the narrow form (byte class of every element, the
+// npool x P byte table of pool[j] * t, the class values), the inverse table
+// and the pool, staged together.  cprod is read in 16-byte units up to its
+// 16-byte padding.
+struct ClsNarrowStage {
+  Staged<3, uint16_t> inv;
+  Staged<1, uint16_t> pool;
+  Staged<3, uint8_t> cls;
+  Staged<1, int32_t> rcq;
+  Staged<10, uint4> tab;
+  __device__ inline void run(const Tables &Tb, uint16_t *s_inv, uint16_t *s_pool, uint8_t *s_cls, int32_t *s_rcq,
+                             uint8_t *s_tab) {
+    const uint32_t n16 = (uint32_t)(pad16((size_t)Tb.npool * Tb.P) / 16);
+    const uint4 *src = reinterpret_cast<const uint4 *>(Tb.cprod);
+    inv.load(Tb.inv, Tb.P);
+    pool.load(Tb.pool, Tb.npool);
+    cls.load(Tb.cls, Tb.P);
+    rcq.load(Tb.rcq, Tb.ncls);
+    tab.load(src, n16);
+    inv.store(s_inv, Tb.inv, Tb.P);
+    pool.store(s_pool, Tb.pool, Tb.npool);
+    cls.store(s_cls, Tb.cls, Tb.P);
+    rcq.store(s_rcq, Tb.rcq, Tb.ncls);
+    tab.store(reinterpret_cast<uint4 *>(s_tab), src, n16);
+  }
+};
+
 // Loads ReTr, the inverse table and the proposal pool of one group into LDS.
 __device__ inline void load_tables_lds(const Tables &Tb, double *s_retr, uint16_t *s_inv, uint16_t *s_pool) {
-  for (uint32_t t = threadIdx.x; t < Tb.P; t += blockDim.x) {
-    s_retr[t] = Tb.retr[t];
-    s_inv[t] = Tb.inv[t];
-  }
-  for (uint32_t t = threadIdx.x; t < Tb.npool; t += blockDim.x) s_pool[t] = Tb.pool[t];
+  Staged<5, double> retr;
+  Staged<5, uint16_t> inv;
+  Staged<2, uint16_t> pool;
+  retr.load(Tb.retr, Tb.P);
+  inv.load(Tb.inv, Tb.P);
+  pool.load(Tb.pool, Tb.npool);
+  retr.store(s_retr, Tb.retr, Tb.P);
+  inv.store(s_inv, Tb.inv, Tb.P);
+  pool.store(s_pool, Tb.pool, Tb.npool);
 }
 
 

@@ -66,16 +66,27 @@ void k_sweep_cls(SweepArgs A) {
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(reinterpret_cast<unsigned char *>(s_inv) + pad16(2 * (size_t)P));
   unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
   unsigned char *s_tab = s_tab0 + pad16((W ? 2 : 1) * (size_t)P);
-  for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) s_inv[t] = A.T.inv[t];
-  for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) s_pool[t] = A.T.pool[t];
-  if (W) {
-    copy16(s_tab0, A.T.cval0, 2 * (size_t)P);
-    copy16(s_tab, A.T.cval, 2 * (size_t)np * P);
+  if constexpr (W) {
+    // 1024 lanes: every table in at most 10 rounds
+    Staged<2, uint16_t> g_inv;
+    Staged<1, uint16_t> g_pool;
+    Staged<1, uint4> g_tab0;
+    Staged<10, uint4> g_tab;
+    const uint32_t n0 = (uint32_t)(pad16(2 * (size_t)P) / 16), n1 = (uint32_t)(pad16(2 * (size_t)np * P) / 16);
+    const uint4 *c0 = reinterpret_cast<const uint4 *>(A.T.cval0), *c1 = reinterpret_cast<const uint4 *>(A.T.cval);
+    g_inv.load(A.T.inv, P);
+    g_pool.load(A.T.pool, np);
+    g_tab0.load(c0, n0);
+    g_tab.load(c1, n1);
+    g_inv.store(s_inv, A.T.inv, P);
+    g_pool.store(s_pool, A.T.pool, np);
+    g_tab0.store(reinterpret_cast<uint4 *>(s_tab0), c0, n0);
+    g_tab.store(reinterpret_cast<uint4 *>(s_tab), c1, n1);
   } else {
-    for (uint32_t t = threadIdx.x; t < nc; t += blockDim.x) s_rcq[t] = A.T.rcq[t];
-    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) s_tab0[t] = A.T.cls[t];
-    copy16(s_tab, A.T.cprod, (size_t)np * P);
+    ClsNarrowStage g;
+    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
   }
+  (void)nc;
   __syncthreads();
 
   const TabView T{A.T.mult, s_inv, P};
@@ -125,13 +136,11 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(smem + pad16(2 * (size_t)P));
   unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
   unsigned char *s_tab = s_tab0 + pad16(P);
-  for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-    s_inv[t] = A.T.inv[t];
-    s_tab0[t] = A.T.cls[t];
+  {
+    ClsNarrowStage g;
+    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
   }
-  for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) s_pool[t] = A.T.pool[t];
-  for (uint32_t t = threadIdx.x; t < nc; t += blockDim.x) s_rcq[t] = A.T.rcq[t];
-  copy16(s_tab, A.T.cprod, (size_t)np * P);
+  (void)nc;
   const uint32_t first = blockIdx.x * NL;
   const uint32_t n = A.count - first < (uint32_t)NL ? A.count - first : (uint32_t)NL;
   const double npd = (double)np;
@@ -183,13 +192,10 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     uint16_t *s_pool = reinterpret_cast<uint16_t *>(smem + pad16(2 * (size_t)P));
     unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
     unsigned char *s_tab = s_tab0 + pad16(P);
-    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-      s_inv[t] = BC.T.inv[t];
-      s_tab0[t] = BC.T.cls[t];
+    {
+      ClsNarrowStage g;
+      g.run(BC.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
     }
-    for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) s_pool[t] = BC.T.pool[t];
-    for (uint32_t t = threadIdx.x; t < BC.T.ncls; t += blockDim.x) s_rcq[t] = BC.T.rcq[t];
-    copy16(s_tab, BC.T.cprod, (size_t)np * P);
     __syncthreads();
     const TabView T{BC.T.mult, s_inv, P};
     const LinkCtx C{BC.links, BC.mstate, nullptr, s_pool, (double)np, BC.beta0, BC.beta1, BC.beta2,

@@ -148,18 +148,35 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
   double *s_retr = reinterpret_cast<double *>(smem);
   uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_retr + P);
   uint16_t *s_pool = s_inv + P;
-  load_tables_lds(BC.T, s_retr, s_inv, s_pool);
   // class path: class of each element and of pool[j] * t, after the element tables
   const bool cls = LC.cls != 0;
   uint8_t *s_cls = smem + pad16(llr_lds_bytes(BC.T, false));
   uint8_t *s_cprod = s_cls + pad16(P);
-  if (cls) {
-    for (uint32_t t = threadIdx.x; t < BC.T.ncls; t += blockDim.x) s_rcq[t] = BC.T.rcq[t];
-    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) s_cls[t] = BC.T.cls[t];
-    const uint32_t n16 = (uint32_t)(pad16((size_t)np * P) / 16);
+  {
+    // every load of the block's staging issued before the first LDS store
+    Staged<5, double> g_retr;
+    Staged<5, uint16_t> g_inv;
+    Staged<1, uint16_t> g_pool;
+    Staged<5, uint8_t> g_cls;
+    Staged<1, int32_t> g_rcq;
+    Staged<10, uint4> g_tab;
+    const uint32_t n16 = cls ? (uint32_t)(pad16((size_t)np * P) / 16) : 0u;
+    const uint32_t ncls = cls ? BC.T.ncls : 0u, pcls = cls ? P : 0u;
     const uint4 *src = reinterpret_cast<const uint4 *>(BC.T.cprod);
-    uint4 *dst = reinterpret_cast<uint4 *>(s_cprod);
-    for (uint32_t t = threadIdx.x; t < n16; t += blockDim.x) dst[t] = src[t];
+    g_retr.load(BC.T.retr, P);
+    g_inv.load(BC.T.inv, P);
+    g_pool.load(BC.T.pool, np);
+    g_cls.load(BC.T.cls, pcls);
+    g_rcq.load(BC.T.rcq, ncls);
+    g_tab.load(src, n16);
+    g_retr.store(s_retr, BC.T.retr, P);
+    g_inv.store(s_inv, BC.T.inv, P);
+    g_pool.store(s_pool, BC.T.pool, np);
+    g_cls.store(s_cls, BC.T.cls, pcls);
+    g_rcq.store(s_rcq, BC.T.rcq, ncls);
+    g_tab.store(reinterpret_cast<uint4 *>(s_cprod), src, n16);
+  }
+  if (cls) {
     const uint32_t sl = (uint32_t)S.sweep, sh = (uint32_t)(S.sweep >> 32), n = tk.count;
     const double npd = (double)np;
     for (uint32_t j = threadIdx.x; j < n * (uint32_t)kNHit; j += blockDim.x) {
