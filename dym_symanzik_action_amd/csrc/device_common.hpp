@@ -234,6 +234,7 @@ struct LinkCtx {
   const double *s_retr;   // LDS
   const uint16_t *s_pool; // LDS
   double npool_d;
+  uint32_t npool;
   double beta0, beta1, beta2;
   uint32_t seed, sweep_lo, sweep_hi;
   int rng;
@@ -262,17 +263,19 @@ __device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const Ta
   if (C.rng == kRngMinstd) ms = C.mstate[i];
   const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
   uint32_t acc = 0;
+  U4 w{0, 0, 0, 0};
   for (int h = 0; h < kNHit; ++h) {
-    double u1, u2;
+    double u2;
+    uint32_t jp;
     if (C.rng == kRngPhilox) {
-      const U4 w = philox4x32_10(U4{link, (uint32_t)h, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
-      u1 = u53(w.x, w.y);
-      u2 = u53(w.z, w.w);
+      if ((h & 1) == 0) w = philox4x32_10(U4{link, (uint32_t)h >> 1, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
+      jp = hit_pool_index((h & 1) ? w.z : w.x, C.npool);
+      u2 = hit_uniform((h & 1) ? w.w : w.y);
     } else {
-      u1 = minstd_canonical(ms);
+      jp = (uint32_t)(C.npool_d * minstd_canonical(ms));
       u2 = minstd_canonical(ms);  // same draw order as the reference: proposal, then acceptance
     }
-    const uint32_t bnew = T.M(b, C.s_pool[(uint32_t)(C.npool_d * u1)]);
+    const uint32_t bnew = T.M(b, C.s_pool[jp]);
     double r1n = 0, m1n = 0;
 #pragma unroll
     for (int j = 0; j < NS; ++j) r1n += C.s_retr[T.M(bnew, st[j])];
@@ -372,23 +375,20 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
   const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
   const double b1q = C.beta1 * L::kScale, b2q = C.beta2 * L::kScale, margin = K.margin;
   uint32_t acc = 0;
+  U4 w{0, 0, 0, 0};
   for (int h = 0; h < kNHit; ++h) {
     double u2;
     uint32_t jp;
     if (pre) {
       jp = pre->jp[h * pre->stride + pre->l];
       u2 = pre->u2[h * pre->stride + pre->l];
+    } else if (C.rng == kRngPhilox) {
+      if ((h & 1) == 0) w = philox4x32_10(U4{link, (uint32_t)h >> 1, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
+      jp = hit_pool_index((h & 1) ? w.z : w.x, C.npool);
+      u2 = hit_uniform((h & 1) ? w.w : w.y);
     } else {
-      double u1;
-      if (C.rng == kRngPhilox) {
-        const U4 w = philox4x32_10(U4{link, (uint32_t)h, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
-        u1 = u53(w.x, w.y);
-        u2 = u53(w.z, w.w);
-      } else {
-        u1 = minstd_canonical(ms);
-        u2 = minstd_canonical(ms);
-      }
-      jp = (uint32_t)(C.npool_d * u1);
+      jp = (uint32_t)(C.npool_d * minstd_canonical(ms));
+      u2 = minstd_canonical(ms);
     }
     int32_t r1n, m1n;
     sums(Lk.row(jp, T.P), r1n, m1n);

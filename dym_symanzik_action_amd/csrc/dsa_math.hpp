@@ -56,6 +56,16 @@ DSA_HD inline double u53(uint32_t hi, uint32_t lo) {
   return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * 0x1.0p-53;
 }
 
+// The Metropolis sweep's Philox stream: one Philox4x32-10 call, counter
+// (link, h >> 1, sweep_lo, sweep_hi) and key (seed, kKeySweep), serves hits h
+// and h + 1 (h even); hit h takes words 2 (h & 1) and 2 (h & 1) + 1 as its
+// proposal index floor(npool w / 2^32) and its acceptance uniform w' / 2^32.
+DSA_HD inline uint32_t hit_pool_index(uint32_t w, uint32_t npool) {
+  return (uint32_t)(((uint64_t)w * npool) >> 32);
+}
+DSA_HD inline double hit_uniform(uint32_t w) { return (double)w * 0x1.0p-32; }
+static_assert(kNHit % 2 == 0, "hits are drawn in pairs");
+
 // std::minstd_rand0 step: x <- 16807 x mod (2^31-1), via the Mersenne fold.
 DSA_HD inline uint32_t minstd_next(uint32_t &x) {
   const uint64_t p = (uint64_t)x * 16807u;

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs A) {
   __syncthreads();
 
   const TabView T{A.T.mult, s_inv, P};
-  const LinkCtx C{A.links, A.mstate, s_retr, s_pool, (double)A.T.npool, A.beta0, A.beta1, A.beta2,
+  const LinkCtx C{A.links, A.mstate, s_retr, s_pool, (double)A.T.npool, A.T.npool, A.beta0, A.beta1, A.beta2,
                   A.seed, A.sweep_lo, A.sweep_hi, A.rng};
   unsigned long long acc = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -47,13 +47,6 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs A) {
 // one lookup per term, one 1024-lane block per CU.  The staples and accepted
 // moves still gather from the global table.
 constexpr int kClsBlockNarrow = 512, kClsBlockWide = 1024;
-
-__device__ inline void copy16(void *dst, const void *src, size_t bytes) {
-  const uint32_t n16 = (uint32_t)(pad16(bytes) / 16);
-  const uint4 *s = reinterpret_cast<const uint4 *>(src);
-  uint4 *d = reinterpret_cast<uint4 *>(dst);
-  for (uint32_t t = threadIdx.x; t < n16; t += blockDim.x) d[t] = s[t];
-}
 
 template <int D, bool W>
 __global__ __launch_bounds__(W ? kClsBlockWide : kClsBlockNarrow) __attribute__((amdgpu_waves_per_eu(4, 8)))
@@ -90,7 +83,7 @@ void k_sweep_cls(SweepArgs A) {
   __syncthreads();
 
   const TabView T{A.T.mult, s_inv, P};
-  const LinkCtx C{A.links, A.mstate, nullptr, s_pool, (double)np, A.beta0, A.beta1, A.beta2,
+  const LinkCtx C{A.links, A.mstate, nullptr, s_pool, (double)np, np, A.beta0, A.beta1, A.beta2,
                   A.seed, A.sweep_lo, A.sweep_hi, A.rng};
   const ClsCtx K{A.T.retr, A.cmargin};
   using L = typename std::conditional<W, ClsWide, ClsNarrow>::type;
@@ -144,17 +137,19 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
   const uint32_t first = blockIdx.x * NL;
   const uint32_t n = A.count - first < (uint32_t)NL ? A.count - first : (uint32_t)NL;
   const double npd = (double)np;
-  for (uint32_t j = threadIdx.x; j < n * (uint32_t)kNHit; j += blockDim.x) {
-    const uint32_t l = j % n, h = j / n;
+  for (uint32_t j = threadIdx.x; j < n * (uint32_t)(kNHit / 2); j += blockDim.x) {
+    const uint32_t l = j % n, q = j / n;  // hits 2q and 2q + 1 of link l
     const uint32_t lk = A.sites[first + l] * (uint32_t)D + (uint32_t)A.mu;
-    const U4 w = philox4x32_10(U4{lk, h, A.sweep_lo, A.sweep_hi}, A.seed, kKeySweep);
-    s_jp[h * NL + l] = (uint16_t)(npd * u53(w.x, w.y));
-    s_u2[h * NL + l] = u53(w.z, w.w);
+    const U4 w = philox4x32_10(U4{lk, q, A.sweep_lo, A.sweep_hi}, A.seed, kKeySweep);
+    s_jp[2 * q * NL + l] = (uint16_t)hit_pool_index(w.x, np);
+    s_u2[2 * q * NL + l] = hit_uniform(w.y);
+    s_jp[(2 * q + 1) * NL + l] = (uint16_t)hit_pool_index(w.z, np);
+    s_u2[(2 * q + 1) * NL + l] = hit_uniform(w.w);
   }
   __syncthreads();
 
   const TabView T{A.T.mult, s_inv, P};
-  const LinkCtx C{A.links, A.mstate, nullptr, s_pool, npd, A.beta0, A.beta1, A.beta2,
+  const LinkCtx C{A.links, A.mstate, nullptr, s_pool, npd, np, A.beta0, A.beta1, A.beta2,
                   A.seed, A.sweep_lo, A.sweep_hi, A.rng};
   const ClsCtx K{A.T.retr, A.cmargin};
   const ClsNarrow Lk{s_tab, s_tab0, s_rcq};
@@ -198,7 +193,7 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     }
     __syncthreads();
     const TabView T{BC.T.mult, s_inv, P};
-    const LinkCtx C{BC.links, BC.mstate, nullptr, s_pool, (double)np, BC.beta0, BC.beta1, BC.beta2,
+    const LinkCtx C{BC.links, BC.mstate, nullptr, s_pool, (double)np, np, BC.beta0, BC.beta1, BC.beta2,
                     BC.seed, (uint32_t)sweep, (uint32_t)(sweep >> 32), BC.rng};
     const ClsCtx K{BC.T.retr, BC.cmargin};
     const ClsNarrow Lk{s_tab, s_tab0, s_rcq};
@@ -214,7 +209,7 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     load_tables_lds(BC.T, s_retr, s_inv, s_pool);
     __syncthreads();
     const TabView T{BC.T.mult, s_inv, P};
-    const LinkCtx C{BC.links, BC.mstate, s_retr, s_pool, (double)np, BC.beta0, BC.beta1, BC.beta2,
+    const LinkCtx C{BC.links, BC.mstate, s_retr, s_pool, (double)np, np, BC.beta0, BC.beta1, BC.beta2,
                     BC.seed, (uint32_t)sweep, (uint32_t)(sweep >> 32), BC.rng};
     for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
       acc += update_link<D>(C, BC.G, T, tk.mu, BC.sites[tk.offset + t]);

@@ -520,9 +520,10 @@ void og_chain_free(og_chain *c) {
   memset(c, 0, sizeof *c);
 }
 
-/* dym-mod-metro.cpp:161-192 for link (i,d).  philox: hit h of sweep s draws
- * Philox(key=(seed,"DSA1"), ctr=(i*D+d, h, s_lo, s_hi)); words 0,1 pick the
- * proposal, words 2,3 the acceptance uniform. */
+/* dym-mod-metro.cpp:161-192 for link (i,d).  philox: hits h and h+1 (h even)
+ * of sweep s share Philox(key=(seed,"DSA1"), ctr=(i*D+d, h/2, s_lo, s_hi));
+ * hit h takes words 2(h&1), 2(h&1)+1: proposal pool[floor(npool w/2^32)] and
+ * acceptance uniform w'/2^32. */
 void og_update_link(og_chain *c, unsigned i, unsigned d) {
   const og_lat *L = &c->L;
   const og_group *g = L->g;
@@ -536,20 +537,23 @@ void og_update_link(og_chain *c, unsigned i, unsigned d) {
   double r1 = 0, m1 = 0;
   for (int j = 0; j < ns; ++j) r1 += g->retr[MUL(b, st[j])];
   for (int k = 0; k < nr; ++k) m1 += g->retr[MUL(b, re[k])];
+  uint32_t o[4] = {0, 0, 0, 0};
   for (int h = 0; h < OG_NHIT; ++h) {
-    double u1, u2;
-    uint32_t o[4];
+    double u2;
+    size_t jp;
     if (c->rng == OG_RNG_PHILOX) {
-      uint32_t ctr[4] = {(uint32_t)l, (uint32_t)h, (uint32_t)c->sweep, (uint32_t)(c->sweep >> 32)};
-      uint32_t key[2] = {(uint32_t)c->seed, OG_KEY_SWEEP};
-      og_philox4x32_10(ctr, key, o);
-      u1 = og_u53(o[0], o[1]);
-      u2 = og_u53(o[2], o[3]);
+      if ((h & 1) == 0) {
+        uint32_t ctr[4] = {(uint32_t)l, (uint32_t)(h >> 1), (uint32_t)c->sweep, (uint32_t)(c->sweep >> 32)};
+        uint32_t key[2] = {(uint32_t)c->seed, OG_KEY_SWEEP};
+        og_philox4x32_10(ctr, key, o);
+      }
+      jp = (size_t)(((uint64_t)o[2 * (h & 1)] * g->npool) >> 32);
+      u2 = (double)o[2 * (h & 1) + 1] * 0x1.0p-32;
     } else {
-      u1 = og_canonical(&c->ms[i]);
+      jp = (size_t)((double)g->npool * og_canonical(&c->ms[i]));
       u2 = 0; /* drawn below, after the proposal's action (same stream order) */
     }
-    int32_t bnew = MUL(b, g->pool[(size_t)((double)g->npool * u1)]);
+    int32_t bnew = MUL(b, g->pool[jp]);
     double r1n = 0, m1n = 0;
     for (int j = 0; j < ns; ++j) r1n += g->retr[MUL(bnew, st[j])];
     for (int k = 0; k < nr; ++k) m1n += g->retr[MUL(bnew, re[k])];

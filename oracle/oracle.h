@@ -15,7 +15,7 @@
  *   minstd-compat: std::minstd_rand0 per site with libstdc++ 11
  *     generate_canonical<double,53> and uniform_int_distribution (downscaling),
  *     as the reference consumes them (dym-mod-metro.cpp:148,174,187,227-232,255);
- *   philox: Philox4x32-10 (Salmon et al., SC'11), one call per hit.
+ *   philox: Philox4x32-10 (Salmon et al., SC'11), one call per two hits.
  * and two visiting orders:
  *   reference: direction -> site parity -> ascending site (dym-mod-metro.cpp:152-159)
  *   colour:    direction -> colour -> site, with a colouring in which no two
