@@ -10,10 +10,14 @@ device before the timed region.
 Prints one JSON line.  Besides the contract fields it carries
   roofline      the sweep kernel against HBM (66 algorithmic bytes of link
                 traffic per link-update), with per-launch times from HIP
-                events on the chain's stream, and its random-gather ceiling
-                (table gathers per link-update vs a measured probe: 608 for
-                the table kernel; 85 + 25 per accepted hit + 98 per exact
-                re-decision for the class kernel);
+                events on the chain's stream; against the L2 request rate
+                that its table gathers meet (L1->L2 read requests per launch
+                from the committed counters vs one request per L2 channel
+                per clock); and against a random-gather probe (table gathers
+                per link-update: 608 for the table kernel; 85 + 25 per
+                accepted hit + 98 per exact re-decision for the class
+                kernel; the probe's lanes gather within one table row per
+                round, so part of its rate is served by L1);
   cpu_baseline  the reference binary (oracle/_ref/dym-mod-metro, built from
                 the reference's sources) timed on this host's cores for two
                 sweeps of the same workload, by its own TM_END: update line.
@@ -44,6 +48,7 @@ GATHERS_CLASS_ACCEPT = 25   # new link + 24 staple products moved by the accepte
 GATHERS_CLASS_EXACT = 98    # b', b^-1 and 4 per staple for an exact re-decision
 KERNEL_NAME = {"table": "k_sweep<4>", "class": "k_sweep_cls<4, true>", "class-narrow": "k_sweep_cls<4, false>"}
 HBM_PEAK_GBS = 8000.0
+L2_REQ_PEAK = 16 * 8 * 2.4e9   # L2 requests/s: 16 channels per XCD, 8 XCDs, 2.4 GHz
 PMC_FILE = {"table": "k_sweep_32x64_pmc.txt", "class-narrow": "k_sweep_cls_narrow_32x64_pmc.txt"}
 
 
@@ -51,14 +56,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(form):
-    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary
-    of the kernel form that ran (None where none is committed)."""
+def read_profile(form, key):
+    """A per-launch figure (HBM bytes, L1->L2 read requests, L2 hit rate) from
+    the committed rocprofv3 --pmc summary of the kernel form that ran (None
+    where none is committed)."""
     if form not in PMC_FILE:
         return None
     try:
         for line in open(os.path.join(REPO, "profiles", PMC_FILE[form])):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -258,7 +264,9 @@ def main():
     avg_launch_s = ms.value * 1e-3 / nl.value
     units = LINKS // npass
     achieved = BYTES_PER_LINK_UPDATE * units / avg_launch_s / 1e9
-    traffic = read_traffic(form)
+    traffic = read_profile(form, "traffic_bytes_per_launch")
+    l2_req = read_profile(form, "l2_read_requests_per_launch")
+    l2_hit = read_profile(form, "l2_hit_rate")
     if form == "table":
         gathers_per_unit = float(GATHERS_TABLE)
     else:
@@ -327,6 +335,14 @@ def main():
                                 # the rate it would have to sustain for this throughput
                                 "reference_equivalent_per_s": GATHERS_TABLE * value,
                                 "reference_equivalent_frac": GATHERS_TABLE * value / best if best else None},
+                     # the bound the staple-product gathers meet: L1->L2 read
+                     # requests (one 128-byte line each) per launch from the
+                     # committed counters, over the live launch time, against
+                     # one request per L2 channel per clock (16 x 8 XCDs x 2.4 GHz)
+                     "l2_requests": {"achieved_per_s": l2_req / avg_launch_s if l2_req else None,
+                                     "peak_per_s": L2_REQ_PEAK,
+                                     "frac": l2_req / avg_launch_s / L2_REQ_PEAK if l2_req else None,
+                                     "per_launch": l2_req, "l2_hit_rate": l2_hit},
                      "exact_redecisions_per_hit": (ex1 - ex0) / (hit1 - hit0)},
         "cpu_baseline": cpu,
         "batched_vs_serial": scan,
