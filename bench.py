@@ -46,7 +46,8 @@ GATHERS_TABLE = 608     # 84 staple + 24 initial + 20*25 hit-loop table gathers
 GATHERS_CLASS = 85      # the 24 staple-times-link products, built directly; per hit: LDS only
 GATHERS_CLASS_ACCEPT = 25   # new link + 24 staple products moved by the accepted factor
 GATHERS_CLASS_EXACT = 98    # b', b^-1 and 4 per staple for an exact re-decision
-KERNEL_NAME = {"table": "k_sweep<4>", "class": "k_sweep_cls<4, true>", "class-narrow": "k_sweep_cls<4, false>"}
+KERNEL_NAME = {"table": "k_sweep<4>", "class": "k_sweep_cls<4, true>", "class-narrow": "k_sweep_cls<4, false>",
+               "matrix": "k_sweep_mat<4>"}
 HBM_PEAK_GBS = 8000.0
 L2_REQ_PEAK = 16 * 8 * 2.4e9   # L2 requests/s: 16 channels per XCD, 8 XCDs, 2.4 GHz
 PMC_FILE = {"table": "k_sweep_32x64_pmc.txt", "class-narrow": "k_sweep_cls_narrow_32x64_pmc.txt"}
@@ -220,7 +221,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=12,
                     help="LLR window requests for the serving measurement (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=16)
-    ap.add_argument("--kernel", choices=["auto", "table", "class", "class-narrow"], default="auto",
+    ap.add_argument("--kernel", choices=["auto", "table", "class", "class-narrow", "matrix"], default="auto",
                     help="sweep kernel (identical results): element values via L2 gathers, or class values in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy kernel of this size (scale for the FETCH/WRITE_SIZE counters)")
@@ -269,6 +270,10 @@ def main():
     l2_hit = read_profile(form, "l2_hit_rate")
     if form == "table":
         gathers_per_unit = float(GATHERS_TABLE)
+    elif form == "matrix":
+        # the new link of an accepted hit; an exact re-decision builds the 24
+        # staple-times-link products (85) and evaluates them (98)
+        gathers_per_unit = ((acc1 - acc0) + (GATHERS_CLASS + GATHERS_CLASS_EXACT) * (ex1 - ex0)) / (args.steps * LINKS)
     else:
         gathers_per_unit = GATHERS_CLASS + (GATHERS_CLASS_ACCEPT * (acc1 - acc0) +
                                             GATHERS_CLASS_EXACT * (ex1 - ex0)) / (args.steps * LINKS)

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -64,6 +65,10 @@ struct dsa_chain {
   bool narrow_ok = false;
   size_t lds_max = 0;  // LDS bytes per block on the chain's device
   double margin8 = 0, margin16 = 0, margin_scale = 1;
+  // the matrix kernel: runs here (the group has matrices, they fit LDS),
+  // its decision bound and the bound's growth per accepted hit
+  bool mat_ok = false;
+  double mat_m0 = 0, mat_macc = 0;
   // per-launch timing (measurement mode): one event pair per pass
   bool profiling = false;
   std::vector<hipEvent_t> pev;
@@ -87,7 +92,64 @@ dsa::Tables dsa::tables_view(const dsa_group *g, const DeviceTables &t) {
   T.ncls = (uint32_t)g->g.rc.size();
   T.cval = t.cval;
   T.cval0 = t.cval0;
+  T.mat = t.mat;
   return T;
+}
+
+bool dsa::su3_check(const Group &g, std::vector<float> &mat32, double &dev) {
+  const size_t P = g.P;
+  if (g.su3.size() != 18 * P) return false;
+  mat32.assign(20 * P, 0.0f);
+  dev = 0;
+  for (size_t e = 0; e < P; ++e) {
+    double tr = 0;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        const double re = g.su3[18 * e + 2 * (c + 3 * r)], im = g.su3[18 * e + 2 * (c + 3 * r) + 1];
+        mat32[20 * e + 3 * r + c] = (float)re;
+        mat32[20 * e + 9 + 3 * r + c] = (float)im;
+        if (r == c) tr += re;
+      }
+    dev = std::max(dev, std::fabs(g.retr[e] + tr));  // the table's ReTr is minus the real trace
+  }
+  // a representation: rho(a) rho(b) = rho(ab) entry by entry, every pair
+  for (size_t a = 0; a < P; ++a) {
+    const double *A = &g.su3[18 * a];
+    for (size_t b = 0; b < P; ++b) {
+      const double *B = &g.su3[18 * b], *Q = &g.su3[18 * (size_t)g.mult[a * P + b]];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double xr = 0, xi = 0;
+          for (int k = 0; k < 3; ++k) {
+            const double ar = A[2 * (k + 3 * r)], ai = A[2 * (k + 3 * r) + 1];
+            const double br = B[2 * (c + 3 * k)], bi = B[2 * (c + 3 * k) + 1];
+            xr += ar * br - ai * bi;
+            xi += ar * bi + ai * br;
+          }
+          dev = std::max(dev, std::max(std::fabs(xr - Q[2 * (c + 3 * r)]), std::fabs(xi - Q[2 * (c + 3 * r) + 1])));
+        }
+    }
+  }
+  return true;
+}
+
+// The matrix kernel's bound.  With u = 2^-24, every FP32 matrix entry
+// rounded from the file (|e| <= sqrt(3) u in Frobenius norm), every product
+// of 3x3 complex matrices a chain of six FMAs per real part (error <=
+// sqrt(2) gamma_6 |A||B|, Frobenius <= 25.5 u ||A||_F), a staple a chain of
+// at most 5 products, accumulated with weights w_k into S and multiplied by
+// rho(b): ||M_computed - M||_F <= ~170 u W with W = sum |w_k| =
+// 2(D-1)|beta1| + 6(D-1)|beta2|.  dS = Re Tr M - Re Tr(rho(p) M) then errs
+// by at most 2 sqrt(3) ||dM||_F + 54 u W (the 18-FMA trace) + ~10 u W (the
+// other trace, rho(p)'s rounding, the weights' rounding): ~660 u W.  Each
+// accepted hit's product M rho(p) adds ~100 u W.  The bound below takes
+// 2048 u W and 256 u W (safety >= 2.5), plus the file's own deviation from
+// the tables (su3_check) and the rounding of the reference's double sums.
+void dsa::mat_margin(int D, double beta0, double beta1, double beta2, double dev, double &m0, double &macc) {
+  const double u = 0x1p-24, ud = 0x1p-53;
+  const double W = 2.0 * (D - 1) * std::fabs(beta1) + 6.0 * (D - 1) * std::fabs(beta2);
+  m0 = W * (2048.0 * u + 64.0 * dev + 1e-12) + 16.0 * ud * (std::fabs(beta0) + 3.0 * W);
+  macc = W * 256.0 * u;
 }
 
 void dsa::class_margin_terms(const Group &g, int D, double q, double &e1, double &e2, double &f1, double &f2,
@@ -190,6 +252,11 @@ int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
     HIP_TRY(hipMemcpy(t.cval, G.cval.data(), G.cval.size() * sizeof(int16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.cval0, G.cval0.data(), P * sizeof(int16_t), hipMemcpyHostToDevice));
   }
+  if (g->mat_state == 0) g->mat_state = dsa::su3_check(G, g->mat32, g->mat_dev) && g->mat_dev < 1e-9 ? 1 : -1;
+  if (g->mat_state > 0) {
+    HIP_TRY(hipMalloc(&t.mat, g->mat32.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(t.mat, g->mat32.data(), g->mat32.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
   g->dev[device] = t;
   out = t;
   return 0;
@@ -205,6 +272,7 @@ int cls_mode(const dsa_chain *c) {
   switch (c->kernel) {
     case DSA_KERNEL_CLASS: return c->cls_best;
     case DSA_KERNEL_CLASS_NARROW: return dsa::kClsNarrow;
+    case DSA_KERNEL_MATRIX: return dsa::kClsMatrix;
     case DSA_KERNEL_AUTO: return c->narrow_ok && c->margin8 < 1e-3 ? (int)dsa::kClsNarrow : c->cls_best;
     default: return dsa::kClsNone;
   }
@@ -362,14 +430,18 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
       hipMemset(c->d_acc, 0, sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->d_fall, 0, sizeof(unsigned long long)) != hipSuccess)
     return bail(fail(DSA_E_DEVICE, "device upload failed"));
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+    lds_max = 0;
+  c->lds_max = (size_t)lds_max;
+  if (c->T.mat) {
+    c->mat_ok = dsa::sweep_cls_lds_bytes(c->T, dsa::kClsMatrix) <= (size_t)lds_max;
+    dsa::mat_margin((int)p->D, p->beta0, p->beta1, p->beta2, g->mat_dev, c->mat_m0, c->mat_macc);
+  }
   if (c->T.cprod) {
-    int lds_max = 0;
-    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
-      lds_max = 0;
     // quantisation: half a unit of the class values (2^-24 narrow, 2^-13 wide)
     c->margin8 = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2, 0x1p-25);
     c->margin16 = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2, 0x1p-14);
-    c->lds_max = (size_t)lds_max;
     // + the kernels' static class-value array
     c->narrow_ok = dsa::sweep_cls_lds_bytes(c->T, dsa::kClsNarrow) + 256 <= (size_t)lds_max;
     const bool wide_ok = c->T.cval && dsa::sweep_cls_lds_bytes(c->T, dsa::kClsWide) + 256 <= (size_t)lds_max;
@@ -433,7 +505,8 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.lds_max = c->lds_max;
   // the class form this chain runs (kClsNone: the table kernel)
   const int mode = cls_mode(c);
-  A.cmargin = (mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
+  A.cmargin = (mode == dsa::kClsMatrix ? c->mat_m0 : mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
+  A.cmargin_acc = c->mat_macc * c->margin_scale;
   const size_t np = c->S.passes.size();
   if (c->profiling && c->pev.size() < 2 * np) {
     for (size_t q = c->pev.size(); q < 2 * np; ++q) {
@@ -480,15 +553,21 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
 int dsa_chain_kernel_form(dsa_chain *c, int32_t *form) {
   if (!c || !form) return fail(DSA_E_ARG, "dsa_chain_kernel_form: NULL argument");
   const int mode = cls_mode(c);
-  *form = mode == dsa::kClsWide ? DSA_KERNEL_CLASS : mode == dsa::kClsNarrow ? DSA_KERNEL_CLASS_NARROW : DSA_KERNEL_TABLE;
+  *form = mode == dsa::kClsMatrix  ? DSA_KERNEL_MATRIX
+          : mode == dsa::kClsWide  ? DSA_KERNEL_CLASS
+          : mode == dsa::kClsNarrow ? DSA_KERNEL_CLASS_NARROW
+                                    : DSA_KERNEL_TABLE;
   return 0;
 }
 
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale) {
   if (!c) return fail(DSA_E_ARG, "dsa_chain_set_kernel: NULL chain");
   if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS &&
-      kernel != DSA_KERNEL_CLASS_NARROW)
+      kernel != DSA_KERNEL_CLASS_NARROW && kernel != DSA_KERNEL_MATRIX)
     return fail(DSA_E_ARG, "dsa_chain_set_kernel: unknown kernel " + std::to_string(kernel));
+  if (kernel == DSA_KERNEL_MATRIX && !c->mat_ok)
+    return fail(DSA_E_STATE, "dsa_chain_set_kernel: the matrix kernel needs the group's SU(3) matrices (appended to "
+                             "its file) in this device's LDS");
   if (!(margin_scale >= 1.0) || !std::isfinite(margin_scale))
     return fail(DSA_E_ARG, "dsa_chain_set_kernel: margin_scale must be finite and >= 1");
   if (kernel == DSA_KERNEL_CLASS && c->cls_best == dsa::kClsNone)

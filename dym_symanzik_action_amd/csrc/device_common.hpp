@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dsa_math.hpp"
 #include "kernels.hpp"
 
@@ -296,6 +298,57 @@ __device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const Ta
   return acc;
 }
 
+// The Metropolis decision on an estimate dc of the action change dS with
+// |dS - dc| <= M, certified against the exact one, without branches: 1
+// accept, 0 reject, -1 undecided.  Above the band (dc > M): v_exp_f32 (2^x,
+// ~1 ulp) on the float-rounded argument is within 5e-6 relative of exp(-dc)
+// for dc < 60, so decide with 1.5e-5 of slack; and exp(-dS) < exp(-37) <
+// 1e-15 <= u2 rejects.  Inside the band (|dc| <= M) exp(-dS) >= 1 - 2M
+// accepts any smaller u2.  Rarely, the double exp settles what the float
+// one leaves open.
+__device__ inline int metro_certified(double dc, double M, double u2) {
+  const double e = (double)__builtin_amdgcn_exp2f((float)(-fmin(fmax(dc, 0.0), 60.0) * 1.4426950408889634));
+  const double w = 1.02 * M + 0x1p-16;
+  const int above = u2 < e * (1.0 - w) ? 1 : (u2 >= e * (1.0 + w) ? 0 : -1);
+  int dec = dc + M <= 0.0  ? 1
+            : dc - M > 0.0 ? ((dc - M > 37.0 && u2 >= 1e-15) ? 0 : (dc < 60.0 ? above : -1))
+                           : (u2 < 1.0 - (2.02 * M + 0x1p-46) ? 1 : -1);
+  if (dec < 0 && dc - M > 0.0) {
+    const double ed = dsa_exp(-dc);
+    const double wd = 1.02 * M + 0x1p-46;
+    dec = u2 < ed * (1.0 - wd) ? 1 : (u2 >= ed * (1.0 + wd) ? 0 : -1);
+  }
+  return dec;
+}
+
+// The reference's decision of one hit (dym-mod-metro.cpp:176-188) from
+// element values, for proposal b -> b p given t_k = s_k b: s_k = t_k b^-1,
+// so ReTr(b s_k) = retr[b t_k b^-1] and ReTr(b' s_k) = retr[b' t_k b^-1],
+// summed in the reference's order.  1 accept, 0 reject.
+template <int D>
+__device__ inline int exact_hit(const LinkCtx &C, const double *retr, const TabView &T, uint32_t b, uint32_t p,
+                                const uint32_t *t, double u2) {
+  constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
+  const uint32_t bn = T.M(b, p), bi = T.I(b);
+  double e1 = 0, e2 = 0, e1n = 0, e2n = 0;
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const uint32_t tk = t[k];
+    const double v = retr[T.M(T.M(b, tk), bi)], vn = retr[T.M(T.M(bn, tk), bi)];
+    if (k < NS) {
+      e1 += v;
+      e1n += vn;
+    } else {
+      e2 += v;
+      e2n += vn;
+    }
+  }
+  const double oldact = C.beta0 + C.beta1 * e1 + C.beta2 * e2;
+  const double newact = C.beta0 + C.beta1 * e1n + C.beta2 * e2n;
+  const double probrat = (newact <= oldact) ? 1.0 : dsa_exp(-(newact - oldact));
+  return probrat > u2 ? 1 : 0;
+}
+
 // Class-product form of update_link, with identical results.
 //
 // ReTr is a class function, so the trial term ReTr(b' s_k) with b' = b p_j
@@ -395,45 +448,10 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
     // class-valued change of action; the exact one lies within M of it
     const double dc = b1q * ((double)r1n - (double)r1) + b2q * ((double)m1n - (double)m1);
     const double M = margin + 0x1p-50 * fabs(dc);
-    // The decision on class values, certified against the exact one, without
-    // branches: 1 accept, 0 reject, -1 undecided.  Above the band (dc > M):
-    // v_exp_f32 (2^x, ~1 ulp) on the float-rounded argument is within 5e-6
-    // relative of exp(-dc) for dc < 60, so decide with 1.5e-5 of slack; and
-    // exp(-dS) < exp(-37) < 1e-15 <= u2 rejects.  Inside the band (|dc| <= M)
-    // exp(-dS) >= 1 - 2M accepts any smaller u2.
-    const double e = (double)__builtin_amdgcn_exp2f((float)(-fmin(fmax(dc, 0.0), 60.0) * 1.4426950408889634));
-    const double w = 1.02 * M + 0x1p-16;
-    const int above = u2 < e * (1.0 - w) ? 1 : (u2 >= e * (1.0 + w) ? 0 : -1);
-    int dec = dc + M <= 0.0  ? 1
-              : dc - M > 0.0 ? ((dc - M > 37.0 && u2 >= 1e-15) ? 0 : (dc < 60.0 ? above : -1))
-                             : (u2 < 1.0 - (2.02 * M + 0x1p-46) ? 1 : -1);
-    if (dec < 0 && dc - M > 0.0) {  // rare: decide with the double exp
-      const double ed = dsa_exp(-dc);
-      const double wd = 1.02 * M + 0x1p-46;
-      dec = u2 < ed * (1.0 - wd) ? 1 : (u2 >= ed * (1.0 + wd) ? 0 : -1);
-    }
+    int dec = metro_certified(dc, M, u2);
     const uint32_t p = C.s_pool[jp];
     if (dec < 0) {
-      // element values as update_link sums them: s_k = t_k b^-1, so
-      // ReTr(b s_k) = retr[b t_k b^-1] and ReTr(b' s_k) = retr[b' t_k b^-1]
-      const uint32_t bn = T.M(b, p), bi = T.I(b);
-      double e1 = 0, e2 = 0, e1n = 0, e2n = 0;
-#pragma unroll
-      for (int k = 0; k < NL; ++k) {
-        const uint32_t tk = t[k];
-        const double v = K.retr[T.M(T.M(b, tk), bi)], vn = K.retr[T.M(T.M(bn, tk), bi)];
-        if (k < NS) {
-          e1 += v;
-          e1n += vn;
-        } else {
-          e2 += v;
-          e2n += vn;
-        }
-      }
-      const double oldact = C.beta0 + C.beta1 * e1 + C.beta2 * e2;
-      const double newact = C.beta0 + C.beta1 * e1n + C.beta2 * e2n;
-      const double probrat = (newact <= oldact) ? 1.0 : dsa_exp(-(newact - oldact));
-      dec = probrat > u2 ? 1 : 0;
+      dec = exact_hit<D>(C, K.retr, T, b, p, t, u2);
       ++nfall;
     }
     if (dec) {
@@ -443,6 +461,266 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
       r1 = r1n;
       m1 = m1n;
       ++acc;
+    }
+  }
+  if (b != b0) C.links[lp] = (uint16_t)b;
+  if (C.rng == kRngMinstd) C.mstate[i] = ms;
+  return acc;
+}
+
+// Matrix form of the hit loop.  rho(g) is the SU(3) matrix the group file
+// appends to element g (ReTr(g) = -Re Tr rho(g), rho(a) rho(b) = rho(ab);
+// both to ~1e-15, checked on the host).  The action of link b against its
+// staples is beta0 - Re Tr(rho(b) S) with the weighted staple sum S =
+// sum_k w_k rho(s_k) (w = beta1 for plaquettes, beta2 for rectangles), so
+// with M = S rho(b) a proposal b -> b p changes it by
+// dS = Re Tr M - Re Tr(rho(p) M), and an accepted p takes M to M rho(p).  S is
+// built in FP32 from the links' matrices in LDS, 66 complex 3x3 products per
+// link instead of 85 table gathers from L2; dS then carries a rounding error
+// below the host's bound (margin0, plus margin_acc per accepted hit), and a
+// hit whose decision that error could change is decided from element values
+// by exact_hit: the links and counters are update_link's.
+
+struct Cm3 {
+  float re[9], im[9];  // row-major entries
+};
+
+// rho(e) from LDS: 20 floats per element (re[9], im[9], 2 unused).
+__device__ inline Cm3 mat_of(const float *s_mat, uint32_t e) {
+  const float4 *q = reinterpret_cast<const float4 *>(s_mat + 20 * (size_t)e);
+  const float4 a = q[0], b = q[1], c = q[2], d = q[3], f = q[4];
+  return Cm3{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x}, {c.y, c.z, c.w, d.x, d.y, d.z, d.w, f.x, f.y}};
+}
+
+// op(A) op(B), op the identity or the conjugate transpose (DA, DB); each
+// entry's real and imaginary part a chain of six FMAs.
+template <bool DA, bool DB>
+__device__ inline Cm3 mm(const Cm3 &A, const Cm3 &B) {
+  Cm3 C;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float r = 0.f, m = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float ar = DA ? A.re[3 * k + i] : A.re[3 * i + k];
+        const float ai = DA ? -A.im[3 * k + i] : A.im[3 * i + k];
+        const float br = DB ? B.re[3 * j + k] : B.re[3 * k + j];
+        const float bi = DB ? -B.im[3 * j + k] : B.im[3 * k + j];
+        r = fmaf(ar, br, r);
+        r = fmaf(-ai, bi, r);
+        m = fmaf(ar, bi, m);
+        m = fmaf(ai, br, m);
+      }
+      C.re[3 * i + j] = r;
+      C.im[3 * i + j] = m;
+    }
+  return C;
+}
+
+// Y += a X
+__device__ inline void axpy(Cm3 &Y, float a, const Cm3 &X) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    Y.re[q] = fmaf(a, X.re[q], Y.re[q]);
+    Y.im[q] = fmaf(a, X.im[q], Y.im[q]);
+  }
+}
+
+__device__ inline Cm3 scaled(float a, const Cm3 &X) {
+  Cm3 Y;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    Y.re[q] = a * X.re[q];
+    Y.im[q] = a * X.im[q];
+  }
+  return Y;
+}
+
+__device__ inline void addto(Cm3 &Y, const Cm3 &X) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    Y.re[q] += X.re[q];
+    Y.im[q] += X.im[q];
+  }
+}
+
+// Re Tr(A B), a chain of 18 FMAs.
+__device__ inline float retr_prod(const Cm3 &A, const Cm3 &B) {
+  float t = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      t = fmaf(A.re[3 * i + j], B.re[3 * j + i], t);
+      t = fmaf(-A.im[3 * i + j], B.im[3 * j + i], t);
+    }
+  return t;
+}
+
+// The weighted staple sum S of link (x, mu): each staple of build_staples
+// as a chain of products accumulated into S: 14 products per side and
+// transverse direction (84 per link at D = 4).  Scheduling
+// barriers after each product keep the compiler from hoisting every
+// matrix's LDS load ahead (18 registers each): a handful of matrices is live
+// at a time and the kernel fits 128 registers.
+template <int D>
+__device__ inline Cm3 staple_sum(const uint16_t *links, const Geom &G, const Site<D> &s, int mu, const float *s_mat,
+                                 float w1, float w2) {
+  // the element of U_dir(x + a mu + b nu)
+  auto E = [&](int dir, int a, int nu, int bb) { return U<D>(links, G, s, dir, mu, a, nu, bb); };
+  // T <- T op(rho(e)), one product at a time
+  auto step = [&](Cm3 &T, uint32_t e, auto dag) {
+    T = mm<false, decltype(dag)::value>(T, mat_of(s_mat, e));
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  using N = std::false_type;
+  using H = std::true_type;
+  const uint32_t eup = E(mu, 1, mu, 0), eum = E(mu, -1, mu, 0);
+  Cm3 S{};
+#pragma unroll 1
+  for (int j = 0; j < D - 1; ++j) {
+    const int nu = j + (j >= mu ? 1 : 0);
+    // this direction's 20 links, loaded together ahead of the products and
+    // held two 16-bit indices per register
+    const uint32_t q0 = E(nu, 1, nu, 0) | E(nu, 0, nu, 0) << 16, q1 = E(nu, 1, nu, -1) | E(nu, 0, nu, -1) << 16;
+    const uint32_t q2 = E(nu, 1, nu, 1) | E(nu, 0, nu, 1) << 16, q3 = E(nu, 2, nu, 0) | E(nu, -1, nu, 0) << 16;
+    const uint32_t q4 = E(nu, 1, nu, -2) | E(nu, 0, nu, -2) << 16, q5 = E(nu, 2, nu, -1) | E(nu, -1, nu, -1) << 16;
+    const uint32_t q6 = E(mu, 0, nu, 1) | E(mu, 0, nu, -1) << 16, q7 = E(mu, 0, nu, 2) | E(mu, 1, nu, 1) << 16;
+    const uint32_t q8 = E(mu, -1, nu, 1) | E(mu, 0, nu, -2) << 16, q9 = E(mu, 1, nu, -1) | E(mu, -1, nu, -1) << 16;
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t n10 = q0 & 0xffffu, n00 = q0 >> 16, n1m = q1 & 0xffffu, n0m = q1 >> 16;
+    const uint32_t n11 = q2 & 0xffffu, n01 = q2 >> 16, n20 = q3 & 0xffffu, nm0 = q3 >> 16;
+    const uint32_t n1M = q4 & 0xffffu, n0M = q4 >> 16, n2m = q5 & 0xffffu, nmm = q5 >> 16;
+    const uint32_t m01 = q6 & 0xffffu, m0m = q6 >> 16, m02 = q7 & 0xffffu, m11 = q7 >> 16;
+    const uint32_t mm1 = q8 & 0xffffu, m0M = q8 >> 16, m1m = q9 & 0xffffu, mmm = q9 >> 16;
+    Cm3 T;
+    // plaquette up, k2
+    T = mat_of(s_mat, n10);
+    step(T, m01, H{});
+    step(T, n00, H{});
+    axpy(S, w1, T);
+    T = mat_of(s_mat, n10);
+    step(T, m01, H{});
+    step(T, mm1, H{});
+    step(T, nm0, H{});
+    step(T, eum, N{});
+    axpy(S, w2, T);
+    // k0
+    T = mat_of(s_mat, n10);
+    step(T, n11, N{});
+    step(T, m02, H{});
+    step(T, n01, H{});
+    step(T, n00, H{});
+    axpy(S, w2, T);
+    // k1
+    T = mat_of(s_mat, eup);
+    step(T, n20, N{});
+    step(T, m11, H{});
+    step(T, m01, H{});
+    step(T, n00, H{});
+    axpy(S, w2, T);
+    // plaquette down, k5
+    T = mm<true, true>(mat_of(s_mat, n1m), mat_of(s_mat, m0m));
+    __builtin_amdgcn_sched_barrier(0);
+    step(T, n0m, N{});
+    axpy(S, w1, T);
+    T = mm<true, true>(mat_of(s_mat, n1m), mat_of(s_mat, m0m));
+    __builtin_amdgcn_sched_barrier(0);
+    step(T, mmm, H{});
+    step(T, nmm, N{});
+    step(T, eum, N{});
+    axpy(S, w2, T);
+    // k3
+    T = mm<true, true>(mat_of(s_mat, n1m), mat_of(s_mat, n1M));
+    __builtin_amdgcn_sched_barrier(0);
+    step(T, m0M, H{});
+    step(T, n0M, N{});
+    step(T, n0m, N{});
+    axpy(S, w2, T);
+    // k4
+    T = mat_of(s_mat, eup);
+    step(T, n2m, H{});
+    step(T, m1m, H{});
+    step(T, m0m, H{});
+    step(T, n0m, N{});
+    axpy(S, w2, T);
+  }
+  return S;
+}
+
+struct MatCtx {
+  const float *s_mat;         // LDS, P x 20 floats
+  const double *retr;         // global, element ReTr for exact decisions
+  double margin0, margin_acc; // |dS - dc| before any accepted hit, and its growth per accepted hit
+};
+
+template <int D>
+__device__ inline uint32_t update_link_mat(const LinkCtx &C, const MatCtx &K, const Geom &G, const TabView &T, int mu,
+                                           uint32_t i, uint32_t &nfall) {
+  constexpr int NL = 8 * (D - 1);
+  const Site<D> s = decode<D>(G, i);
+  const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
+  uint32_t b = C.links[lp];
+  const uint32_t b0 = b;
+  uint32_t ms = 0;
+  if (C.rng == kRngMinstd) ms = C.mstate[i];
+  const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
+  // hit h's proposal index and acceptance uniform, in stream order
+  U4 w{0, 0, 0, 0};
+  auto draw = [&](int h, uint32_t &jp, double &u2) {
+    if (C.rng == kRngPhilox) {
+      if ((h & 1) == 0) w = philox4x32_10(U4{link, (uint32_t)h >> 1, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
+      jp = hit_pool_index((h & 1) ? w.z : w.x, C.npool);
+      u2 = hit_uniform((h & 1) ? w.w : w.y);
+    } else {
+      jp = (uint32_t)(C.npool_d * minstd_canonical(ms));
+      u2 = minstd_canonical(ms);
+    }
+  };
+  uint32_t acc = 0;
+  int h = 0;
+  uint32_t jp = 0;
+  double u2 = 0;
+  {
+    Cm3 M = mm<false, false>(staple_sum<D>(C.links, G, s, mu, K.s_mat, (float)C.beta1, (float)C.beta2),
+                             mat_of(K.s_mat, b));
+    float v0 = M.re[0] + M.re[4] + M.re[8];
+    double margin = K.margin0;
+    for (; h < kNHit; ++h) {
+      draw(h, jp, u2);
+      const uint32_t p = C.s_pool[jp];
+      const Cm3 R = mat_of(K.s_mat, p);
+      // dS = Re Tr M - Re Tr(rho(p) M); the difference of two floats is exact in double
+      const double dc = (double)v0 - (double)retr_prod(R, M);
+      const int dec = metro_certified(dc, margin + 0x1p-50 * fabs(dc), u2);
+      if (dec < 0) break;  // this hit and the rest from element values, below
+      if (dec) {
+        b = T.M(b, p);
+        M = mm<false, false>(M, R);
+        v0 = M.re[0] + M.re[4] + M.re[8];
+        margin += K.margin_acc;
+        ++acc;
+      }
+    }
+  }
+  if (h < kNHit) {
+    // rare: from the undecided hit on, element values (t_k = s_k b as the
+    // class kernel builds them), hit h's draws already taken
+    uint32_t t[NL];
+    build_t<D>(C.links, G, T, s, mu, b, t);
+    for (;;) {
+      const uint32_t p = C.s_pool[jp];
+      ++nfall;
+      if (exact_hit<D>(C, K.retr, T, b, p, t, u2)) {
+        b = T.M(b, p);
+#pragma unroll
+        for (int k = 0; k < NL; ++k) t[k] = T.M(t[k], p);
+        ++acc;
+      }
+      if (++h >= kNHit) break;
+      draw(h, jp, u2);
     }
   }
   if (b != b0) C.links[lp] = (uint16_t)b;

@@ -4,6 +4,7 @@
 
 #include <map>
 #include <mutex>
+#include <vector>
 #include <string>
 
 #include "../../include/dsa.h"
@@ -21,6 +22,7 @@ struct DeviceTables {
   uint8_t *cls = nullptr, *cprod = nullptr;  // class tables (null if the group has none)
   int32_t *rcq = nullptr;
   int16_t *cval = nullptr, *cval0 = nullptr;
+  float *mat = nullptr;  // FP32 SU(3) matrices, 20 floats per element (null if none)
 };
 
 }  // namespace dsa
@@ -28,6 +30,11 @@ struct DeviceTables {
 struct dsa_group {
   dsa::Group g;
   std::mutex m;
+  // the matrix sweep kernel's FP32 matrices and the worst deviation of the
+  // file's matrices from the tables (su3_check), computed on first upload
+  int mat_state = 0;  // 0 unchecked, 1 usable, -1 not usable
+  std::vector<float> mat32;
+  double mat_dev = 0;
   std::map<int, dsa::DeviceTables> dev;  // one upload per device
 };
 
@@ -46,6 +53,15 @@ double class_margin(const Group &g, int D, double beta0, double beta1, double be
 // sum rounding and magnitude of the plaquette (1) and rectangle (2) sums.
 void class_margin_terms(const Group &g, int D, double q, double &e1, double &e2, double &f1, double &f2, double &r1,
                         double &r2);
+// The matrix kernel's FP32 matrices (20 floats per element: re[9], im[9],
+// 2 zero) and the largest deviation of the file's double matrices from the
+// tables: |ReTr(g) + Re Tr rho(g)| over g and every entry of
+// rho(a) rho(b) - rho(ab) over all pairs.  False when the file has none.
+bool su3_check(const Group &g, std::vector<float> &mat32, double &dev);
+// Bound on |dS_exact - dS_matrix| for the matrix kernel's first hit, and its
+// growth per accepted hit (the FP32 rounding of the staple products, their
+// weighted sum and the traces; see su3 notes in capi.cpp).
+void mat_margin(int D, double beta0, double beta1, double beta2, double dev, double &m0, double &macc);
 // Shape and option checks shared by chains and batches.
 int check_params(const dsa_chain_params *p);
 // minstd-compat start (dym-mod-metro.cpp:227-228, 253-255): per-site states and

@@ -106,6 +106,52 @@ void k_sweep_cls(SweepArgs A) {
   }
 }
 
+// The same pass with the matrix update: the group's SU(3) matrices (80 B
+// per element, 86 KB for S(1080)) in LDS, one 1024-lane block per CU; only
+// the links (and accepted hits' new links) come from global memory.
+constexpr int kMatBlock = 1024;
+
+template <int D>
+__global__ __launch_bounds__(kMatBlock) void k_sweep_mat(SweepArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t P = A.T.P, np = A.T.npool;
+  float *s_mat = reinterpret_cast<float *>(smem);
+  uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem + 80 * (size_t)P);
+  uint16_t *s_pool = reinterpret_cast<uint16_t *>(reinterpret_cast<unsigned char *>(s_inv) + pad16(2 * (size_t)P));
+  {
+    // two rounds of 16-byte loads in flight per batch (a larger staging array
+    // would be placed in scratch memory, which caps the waves in flight)
+    Staged<2, uint4> g_mat;
+    Staged<2, uint16_t> g_inv;
+    Staged<1, uint16_t> g_pool;
+    const uint32_t n16 = 5 * P;
+    const uint4 *src = reinterpret_cast<const uint4 *>(A.T.mat);
+    g_mat.load(src, n16);
+    g_inv.load(A.T.inv, P);
+    g_pool.load(A.T.pool, np);
+    g_mat.store(reinterpret_cast<uint4 *>(s_mat), src, n16);
+    g_inv.store(s_inv, A.T.inv, P);
+    g_pool.store(s_pool, A.T.pool, np);
+  }
+  __syncthreads();
+
+  const TabView T{A.T.mult, s_inv, P};
+  const LinkCtx C{A.links, A.mstate, nullptr, s_pool, (double)np, np, A.beta0, A.beta1, A.beta2,
+                  A.seed, A.sweep_lo, A.sweep_hi, A.rng};
+  const MatCtx K{s_mat, A.T.retr, A.cmargin, A.cmargin_acc};
+  unsigned long long acc = 0;
+  uint32_t nfall = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride)
+    acc += update_link_mat<D>(C, K, A.G, T, A.mu, A.sites[k], nfall);
+  acc = wave_sum_u64(acc);
+  const unsigned long long nf = wave_sum_u64(nfall);
+  if ((threadIdx.x & 63u) == 0) {
+    if (acc) atomicAdd(A.acc, acc);
+    if (nf) atomicAdd(A.fall, nf);
+  }
+}
+
 // The narrow form for passes too small to fill the device: a 512-lane block
 // takes NL links; all its lanes first draw the links' Philox uniforms
 // (512 / NL lanes per link), so a link's hits wait only on its lookups.  NL
@@ -489,6 +535,7 @@ size_t sweep_lds_bytes(const Tables &T) {
 }
 
 size_t sweep_cls_lds_bytes(const Tables &T, int mode) {
+  if (mode == kClsMatrix) return 80 * (size_t)T.P + pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool);
   if (mode == kClsWide)
     return pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool) + pad16(2 * (size_t)T.P) +
            pad16(2 * (size_t)T.npool * T.P);
@@ -521,6 +568,20 @@ hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
       default: return hipErrorInvalidValue;
     }
 #undef DSA_SMALL
+    return hipGetLastError();
+  }
+  if (A.use_cls == kClsMatrix) {
+    if (!A.fall || !A.T.mat) return hipErrorInvalidValue;
+    uint32_t nb = (A.count + kMatBlock - 1) / kMatBlock;
+    if (max_blocks > 0 && nb > (uint32_t)max_blocks) nb = (uint32_t)max_blocks;
+    const size_t lds = sweep_cls_lds_bytes(A.T, kClsMatrix);
+    switch (A.G.D) {
+      case 2: hipLaunchKernelGGL(k_sweep_mat<2>, dim3(nb), dim3(kMatBlock), lds, s, A); break;
+      case 3: hipLaunchKernelGGL(k_sweep_mat<3>, dim3(nb), dim3(kMatBlock), lds, s, A); break;
+      case 4: hipLaunchKernelGGL(k_sweep_mat<4>, dim3(nb), dim3(kMatBlock), lds, s, A); break;
+      case 5: hipLaunchKernelGGL(k_sweep_mat<5>, dim3(nb), dim3(kMatBlock), lds, s, A); break;
+      default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
   }
   if (A.use_cls == kClsNarrow || A.use_cls == kClsWide) {

@@ -34,6 +34,9 @@ struct Tables {
   uint32_t ncls;
   // wide form: class values in units of 2^-13 (null if absent)
   const int16_t *cval, *cval0;
+  // the group's SU(3) matrices in FP32, 20 floats per element (re[9], im[9],
+  // 2 unused), row-major (null where the group file has none)
+  const float *mat;
 };
 
 enum RngKind : int { kRngMinstd = 0, kRngPhilox = 1 };
@@ -51,11 +54,13 @@ struct SweepArgs {
   double beta0, beta1, beta2;
   uint32_t seed;
   uint32_t sweep_lo, sweep_hi;
-  // class-product kernel (kClsNone, kClsNarrow, kClsWide), the bound on
-  // |exact - class| trial-action differences below which a decision is
-  // recomputed exactly, and the count of such recomputations
+  // class-product kernel (kClsNone, kClsNarrow, kClsWide) or the matrix
+  // kernel (kClsMatrix), the bound on |exact - estimated| trial-action
+  // differences below which a decision is recomputed exactly (the matrix
+  // kernel's grows by cmargin_acc per accepted hit), and the count of such
+  // recomputations
   int use_cls;
-  double cmargin;
+  double cmargin, cmargin_acc;
   unsigned long long *fall;
   // the device's LDS bytes per block: the small-pass class forms run only
   // where their draw arrays fit beside the tables
@@ -63,7 +68,7 @@ struct SweepArgs {
 };
 
 size_t sweep_lds_bytes(const Tables &T);
-enum ClsMode : int { kClsNone = 0, kClsNarrow = 1, kClsWide = 2 };
+enum ClsMode : int { kClsNone = 0, kClsNarrow = 1, kClsWide = 2, kClsMatrix = 3 };
 size_t sweep_cls_lds_bytes(const Tables &T, int mode);
 __host__ __device__ inline size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s);

@@ -22,7 +22,7 @@ DSA_RNG = {"philox": 0, "minstd": 1}
 DSA_ORDER = {"colour": 0, "reference": 1}
 DSA_START = {"hot": 0, "cold": 1}
 DSA_SUM = {"tree": 0, "ordered": 1}
-DSA_KERNEL = {"auto": 0, "table": 1, "class": 2, "class-narrow": 3}
+DSA_KERNEL = {"auto": 0, "table": 1, "class": 2, "class-narrow": 3, "matrix": 4}
 LOOP_KIND = {"plaq": 0, "rect": 1, "poly": 2, "wilson": 3}
 
 
@@ -266,13 +266,15 @@ class Chain:
     def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
         """Sweep kernel: "table" (element ReTr via L2 gathers), "class" (class
         values in LDS, exact re-decision near the bound), "class-narrow" (the
-        byte class-index form), or "auto"; all give identical links.  A huge
-        margin_scale re-decides every hit exactly."""
+        byte class-index form), "matrix" (FP32 products of the group's SU(3)
+        matrices in LDS, exact re-decision near the rounding bound; groups
+        whose file carries the matrices), or "auto"; all give identical
+        links.  A huge margin_scale re-decides every hit exactly."""
         _check(lib().dsa_chain_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
 
     def kernel_form(self) -> str:
         """The kernel the chain's passes run: "table", "class" (the int16-value
-        form) or "class-narrow"."""
+        form), "class-narrow" or "matrix"."""
         f = C.c_int32()
         _check(lib().dsa_chain_kernel_form(self._h, C.byref(f)))
         return {v: k for k, v in DSA_KERNEL.items()}[f.value]

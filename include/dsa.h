@@ -145,15 +145,20 @@ int dsa_chain_save_kentucky(dsa_chain *c, const char *prefix, int32_t num, char 
  * (default): the class kernel when the group's class tables fit the
  * device's LDS (it is faster at every lattice size measured, 8^4 to
  * 32^3 x 64), in the byte-index form where that fits, else as CLASS.
+ * MATRIX (groups whose file appends SU(3) matrices, e.g. S(1080)): the
+ * staple sum and the trial actions as FP32 3x3 complex products of those
+ * matrices in LDS instead of table gathers; a hit whose decision the
+ * rounding bound could change is decided again from element values.
  * margin_scale (default 1) multiplies the bound
  * that selects those re-decisions: a huge value makes every hit re-decided
  * (a test of that path). */
-enum { DSA_KERNEL_AUTO = 0, DSA_KERNEL_TABLE = 1, DSA_KERNEL_CLASS = 2, DSA_KERNEL_CLASS_NARROW = 3 };
+enum { DSA_KERNEL_AUTO = 0, DSA_KERNEL_TABLE = 1, DSA_KERNEL_CLASS = 2, DSA_KERNEL_CLASS_NARROW = 3,
+       DSA_KERNEL_MATRIX = 4 };
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale);
-/* The kernel the chain's passes run: TABLE, CLASS (the int16-value form) or
- * CLASS_NARROW (the byte-index form). */
+/* The kernel the chain's passes run: TABLE, CLASS (the int16-value form),
+ * CLASS_NARROW (the byte-index form) or MATRIX. */
 int dsa_chain_kernel_form(dsa_chain *c, int32_t *form);
-/* Hits the CLASS kernel re-decided from element values since creation. */
+/* Hits the CLASS or MATRIX kernel re-decided from element values since creation. */
 int dsa_chain_exact_decisions(dsa_chain *c, uint64_t *n);
 
 /* ---- many chains at once -------------------------------------------------

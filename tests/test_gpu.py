@@ -143,6 +143,63 @@ def test_colour_order_matches_oracle(dsa, dgroups, oracle, ogroups, case, rng, k
         assert np.array_equal(m, o.measure(min(Nx, 3), min(Nt, 3), mode=0))
 
 
+@pytest.mark.parametrize("case", [
+    ("mys1080-v4", 4, 2, 2, 0.0, 1.5, 0.1, 1, "hot", 6),
+    ("mys1080-v4", 4, 4, 4, 0.0, 3.0, -0.25, 7, "hot", 4),
+    ("mys1080-v4", 4, 8, 4, 0.1, 2.0, -0.2, 8, "cold", 3),
+    ("mys1080-v4", 4, 6, 6, 0.0, 2.5, -0.1, 9, "hot", 2),
+    ("mys1080-v4", 4, 5, 4, 0.0, 2.0, -0.1, 14, "hot", 2),
+    ("mys1080-v4", 3, 6, 4, 0.0, 4.0, 0.3, 15, "hot", 3),
+])
+@pytest.mark.parametrize("rng", ["philox", "minstd"])
+@pytest.mark.parametrize("kernel", ["matrix", "matrix-all-exact"])
+def test_matrix_kernel_matches_oracle(dsa, dgroups, oracle, ogroups, case, rng, kernel):
+    """The matrix kernel (FP32 SU(3) products, certified decisions) against
+    the oracle; matrix-all-exact widens its bound so that every hit takes the
+    element-value path."""
+    name, D, Nt, Nx, b0, b1, b2, seed, start, nsw = case
+    say("case", name, D, Nt, Nx, rng, kernel)
+    o = oracle.OChain(ogroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour")
+    with dsa.Chain(dgroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour") as c:
+        c.set_kernel("matrix", 1e300 if kernel == "matrix-all-exact" else 1.0)
+        assert c.kernel_form() == "matrix"
+        for s in range(nsw):
+            c.update(1)
+            o.sweep(1)
+            assert np.array_equal(c.a, o.a), f"links differ after sweep {s + 1}"
+        assert c.counters() == o.counters()
+        if rng == "minstd":
+            assert np.array_equal(c.rng_state(), o.mstate())
+        if kernel == "matrix-all-exact":
+            assert c.exact_decisions() == c.counters()[0]
+
+
+def test_matrix_kernel_needs_matrices(dsa, dgroups):
+    """Groups whose file carries no SU(3) matrices refuse the matrix kernel."""
+    say("case matrix kernel refused for S216")
+    with dsa.Chain(dgroups("S216"), 4, 4, 4, 0.0, 2.0, -0.25, 3) as c:
+        with pytest.raises(dsa.DSAError):
+            c.set_kernel("matrix")
+
+
+@pytest.mark.parametrize("Nt,Nx", [(32, 16), (24, 24)])
+def test_matrix_kernel_equals_class_kernel(dsa, dgroups, Nt, Nx):
+    """16^3 x 32 and 24^4 (passes of 65,536 and 41,472 links): the matrix
+    kernel's trajectory is the class kernel's, bit for bit, over two sweeps,
+    and it re-decides few hits from element values."""
+    say("case matrix vs class", Nt, Nx)
+    g = dgroups("mys1080-v4")
+    out = []
+    for kernel in ("class-narrow", "matrix"):
+        with dsa.Chain(g, 4, Nt, Nx, 0.0, 3.0, -0.25, 22) as c:
+            c.set_kernel(kernel)
+            c.update(2)
+            out.append((c.a, c.counters(), c.exact_decisions()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+    assert out[1][2] < 1e-2 * out[1][1][0]
+
+
 def test_config2_shape_one_sweep_matches_oracle(dsa, dgroups, oracle, ogroups):
     """16^3 x 32 (524,288 links), S1080, hot seed 7, beta = (0, 3.0, -0.25)."""
     say("case 16^3x32 one sweep vs oracle")
