@@ -468,6 +468,45 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
     const int raw = mine ? (int)LC.nacc[idx] : 0;
     const bool changed = (raw & kChanged) != 0;
     const int dn = changed ? raw & ~kChanged : 0;  // hits of proposals that moved their link
+    if (R.drive && R.axis != 3) {
+      // Drive-in on one window (axes 1, 2) with the energy E on one side of
+      // the centre: a proposal gets strictly closer iff it moves toward the
+      // centre and lands no further beyond it than E was before it.  If
+      // accepting every changed proposal that moves toward the centre leaves
+      // the energy on the same side (or at the centre), each of them met an
+      // energy at least its own change away from the centre: the sequential
+      // rule accepts exactly those and refuses the others, and the segment
+      // is settled with one block-wide sum instead of the walk below.
+      const long long E = R.axis == 1 ? E1 : E2;
+      const int sgn = E < R.mid ? 1 : (E > R.mid ? -1 : 0);  // uniform over the block
+      const long long dw = R.axis == 1 ? d1 : d2;
+      const bool toward = changed && (sgn > 0 ? dw > 0 : (sgn < 0 && dw < 0));
+      long long a1 = wave_sum_i64(toward ? d1 : 0), a2 = wave_sum_i64(toward ? d2 : 0);
+      const long long an = wave_sum_i64(toward ? dn : 0);
+      if (lane == 0) {
+        sW1[wv] = a1;
+        sW2[wv] = a2;
+        sWN[wv] = (int)an;
+      }
+      __syncthreads();
+      a1 = 0;
+      a2 = 0;
+      long long nt = 0;
+      for (int w = 0; w < kDecideWaves; ++w) {
+        a1 += sW1[w];
+        a2 += sW2[w];
+        nt += sWN[w];
+      }
+      __syncthreads();  // the wave slots are written again below
+      const long long Eend = E + (R.axis == 1 ? a1 : a2);
+      if (sgn != 0 && (sgn > 0 ? Eend <= R.mid : Eend >= R.mid)) {
+        if (changed && !toward) links[LC.pos[idx]] = LC.old[idx];
+        E1 += a1;
+        E2 += a2;
+        acc += (unsigned long long)nt;
+        continue;
+      }
+    }
     long long p1 = scan_i64(d1, lane), p2 = scan_i64(d2, lane);
     int pn = scan_i32(dn, lane);
     const int pt = scan_i32(changed ? 0 : raw, lane);  // hits of the others
