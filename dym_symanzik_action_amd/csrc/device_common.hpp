@@ -735,6 +735,10 @@ __device__ inline uint32_t update_link_mat(const LinkCtx &C, const MatCtx &K, co
 // block's staging behind one round trip to L2; a load-store loop waits on
 // each load before the store of the same iteration (one round trip per
 // iteration, ~18 for the S(1080) class tables at 512 lanes).
+// 16-byte unit of the staged copies: a native vector type, which the
+// compiler keeps in registers (arrays of HIP's uint4 struct land in scratch).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <int K, typename T>
 struct Staged {
   T r[K];
@@ -764,11 +768,11 @@ struct ClsNarrowStage {
   Staged<1, uint16_t> pool;
   Staged<3, uint8_t> cls;
   Staged<1, int32_t> rcq;
-  Staged<10, uint4> tab;
+  Staged<10, u32x4> tab;
   __device__ inline void run(const Tables &Tb, uint16_t *s_inv, uint16_t *s_pool, uint8_t *s_cls, int32_t *s_rcq,
                              uint8_t *s_tab) {
     const uint32_t n16 = (uint32_t)(pad16((size_t)Tb.npool * Tb.P) / 16);
-    const uint4 *src = reinterpret_cast<const uint4 *>(Tb.cprod);
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(Tb.cprod);
     inv.load(Tb.inv, Tb.P);
     pool.load(Tb.pool, Tb.npool);
     cls.load(Tb.cls, Tb.P);
@@ -778,7 +782,7 @@ struct ClsNarrowStage {
     pool.store(s_pool, Tb.pool, Tb.npool);
     cls.store(s_cls, Tb.cls, Tb.P);
     rcq.store(s_rcq, Tb.rcq, Tb.ncls);
-    tab.store(reinterpret_cast<uint4 *>(s_tab), src, n16);
+    tab.store(reinterpret_cast<u32x4 *>(s_tab), src, n16);
   }
 };
 

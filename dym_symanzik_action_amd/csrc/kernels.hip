@@ -63,18 +63,18 @@ void k_sweep_cls(SweepArgs A) {
     // 1024 lanes: every table in at most 10 rounds
     Staged<2, uint16_t> g_inv;
     Staged<1, uint16_t> g_pool;
-    Staged<1, uint4> g_tab0;
-    Staged<10, uint4> g_tab;
+    Staged<1, u32x4> g_tab0;
+    Staged<10, u32x4> g_tab;
     const uint32_t n0 = (uint32_t)(pad16(2 * (size_t)P) / 16), n1 = (uint32_t)(pad16(2 * (size_t)np * P) / 16);
-    const uint4 *c0 = reinterpret_cast<const uint4 *>(A.T.cval0), *c1 = reinterpret_cast<const uint4 *>(A.T.cval);
+    const u32x4 *c0 = reinterpret_cast<const u32x4 *>(A.T.cval0), *c1 = reinterpret_cast<const u32x4 *>(A.T.cval);
     g_inv.load(A.T.inv, P);
     g_pool.load(A.T.pool, np);
     g_tab0.load(c0, n0);
     g_tab.load(c1, n1);
     g_inv.store(s_inv, A.T.inv, P);
     g_pool.store(s_pool, A.T.pool, np);
-    g_tab0.store(reinterpret_cast<uint4 *>(s_tab0), c0, n0);
-    g_tab.store(reinterpret_cast<uint4 *>(s_tab), c1, n1);
+    g_tab0.store(reinterpret_cast<u32x4 *>(s_tab0), c0, n0);
+    g_tab.store(reinterpret_cast<u32x4 *>(s_tab), c1, n1);
   } else {
     ClsNarrowStage g;
     g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
@@ -119,17 +119,15 @@ __global__ __launch_bounds__(kMatBlock) void k_sweep_mat(SweepArgs A) {
   uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem + 80 * (size_t)P);
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(reinterpret_cast<unsigned char *>(s_inv) + pad16(2 * (size_t)P));
   {
-    // two rounds of 16-byte loads in flight per batch (a larger staging array
-    // would be placed in scratch memory, which caps the waves in flight)
-    Staged<2, uint4> g_mat;
+    Staged<6, u32x4> g_mat;
     Staged<2, uint16_t> g_inv;
     Staged<1, uint16_t> g_pool;
     const uint32_t n16 = 5 * P;
-    const uint4 *src = reinterpret_cast<const uint4 *>(A.T.mat);
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(A.T.mat);
     g_mat.load(src, n16);
     g_inv.load(A.T.inv, P);
     g_pool.load(A.T.pool, np);
-    g_mat.store(reinterpret_cast<uint4 *>(s_mat), src, n16);
+    g_mat.store(reinterpret_cast<u32x4 *>(s_mat), src, n16);
     g_inv.store(s_inv, A.T.inv, P);
     g_pool.store(s_pool, A.T.pool, np);
   }

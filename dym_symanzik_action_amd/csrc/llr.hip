@@ -159,10 +159,10 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     Staged<1, uint16_t> g_pool;
     Staged<5, uint8_t> g_cls;
     Staged<1, int32_t> g_rcq;
-    Staged<10, uint4> g_tab;
+    Staged<10, u32x4> g_tab;
     const uint32_t n16 = cls ? (uint32_t)(pad16((size_t)np * P) / 16) : 0u;
     const uint32_t ncls = cls ? BC.T.ncls : 0u, pcls = cls ? P : 0u;
-    const uint4 *src = reinterpret_cast<const uint4 *>(BC.T.cprod);
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(BC.T.cprod);
     g_retr.load(BC.T.retr, P);
     g_inv.load(BC.T.inv, P);
     g_pool.load(BC.T.pool, np);
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     g_pool.store(s_pool, BC.T.pool, np);
     g_cls.store(s_cls, BC.T.cls, pcls);
     g_rcq.store(s_rcq, BC.T.rcq, ncls);
-    g_tab.store(reinterpret_cast<uint4 *>(s_cprod), src, n16);
+    g_tab.store(reinterpret_cast<u32x4 *>(s_cprod), src, n16);
   }
   if (cls) {
     const uint32_t sl = (uint32_t)S.sweep, sh = (uint32_t)(S.sweep >> 32), n = tk.count;
