@@ -460,12 +460,33 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
   uint16_t *links = LC.base.links;
   if (tid < 3) slots[tid] = 0xffffffffu;
   uint32_t k = 0;
+  // a segment's per-proposal inputs, loaded one segment ahead so that the
+  // next segment's loads are in flight while this one is decided
+  struct In {
+    long long d1, d2;
+    int raw;
+    uint32_t pos;
+    uint16_t old;
+  };
+  auto load_in = [&](uint32_t seg) {
+    In v{0, 0, 0, 0, 0};
+    if (seg < pr.count && tid < pr.count - seg) {
+      const uint32_t idx = pr.offset + seg + tid;
+      v.d1 = LC.dE1[idx];
+      v.d2 = LC.dE2[idx];
+      v.raw = (int)LC.nacc[idx];
+      v.pos = LC.pos[idx];
+      v.old = LC.old[idx];
+    }
+    return v;
+  };
+  In nxt = load_in(0);
   for (uint32_t seg = 0; seg < pr.count; seg += kDecide) {
     const uint32_t n = pr.count - seg < (uint32_t)kDecide ? pr.count - seg : (uint32_t)kDecide;
-    const bool mine = tid < n;
-    const uint32_t idx = pr.offset + seg + (mine ? tid : 0);
-    const long long d1 = mine ? LC.dE1[idx] : 0, d2 = mine ? LC.dE2[idx] : 0;
-    const int raw = mine ? (int)LC.nacc[idx] : 0;
+    const In cur = nxt;
+    nxt = load_in(seg + kDecide);
+    const long long d1 = cur.d1, d2 = cur.d2;
+    const int raw = cur.raw;
     const bool changed = (raw & kChanged) != 0;
     const int dn = changed ? raw & ~kChanged : 0;  // hits of proposals that moved their link
     if (R.drive && R.axis != 3) {
@@ -500,7 +521,7 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
       __syncthreads();  // the wave slots are written again below
       const long long Eend = E + (R.axis == 1 ? a1 : a2);
       if (sgn != 0 && (sgn > 0 ? Eend <= R.mid : Eend >= R.mid)) {
-        if (changed && !toward) links[LC.pos[idx]] = LC.old[idx];
+        if (changed && !toward) links[cur.pos] = cur.old;
         E1 += a1;
         E2 += a2;
         acc += (unsigned long long)nt;
@@ -516,8 +537,8 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
       sWN[wv] = pn;
       sWT[wv] = pt;
     }
-    sOld[tid] = LC.old[idx];
-    sPos[tid] = LC.pos[idx];
+    sOld[tid] = cur.old;
+    sPos[tid] = cur.pos;
     sChg[tid] = changed ? 1 : 0;
     __syncthreads();
     for (int w = 0; w < wv; ++w) {
