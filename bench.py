@@ -9,8 +9,9 @@ device before the timed region.
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline      the sweep kernel against HBM (66 algorithmic bytes of link
-                traffic per link-update), with per-launch times from HIP
-                events on the chain's stream; against the L2 request rate
+                traffic per link-update), with the time per launch from a
+                HIP event pair on the chain's stream around the timed sweeps;
+                against the L2 request rate
                 that its table gathers meet (L1->L2 read requests per launch
                 from the committed counters vs one request per L2 channel
                 per clock); and against a random-gather probe (table gathers
@@ -248,9 +249,11 @@ def main():
     hit0, acc0 = c.counters()
     ex0 = c.exact_decisions()
     log(f"bench: timing {args.steps} sweeps")
-    # every pass launch of the timed sweeps is bracketed by HIP events on the
-    # chain's stream: the roofline's per-launch time comes from the same sweeps
-    dsa.lib().dsa_chain_set_profiling(c._h, 1)
+    # one HIP event pair on the chain's stream around the timed sweeps: the
+    # roofline's time per launch is their event time over the pass launches
+    # (gaps between launches included).  Events around every launch would
+    # order a cache release between passes and slow the sweep by ~9 %.
+    dsa.lib().dsa_chain_set_profiling(c._h, 2)
     t0 = time.perf_counter()
     c.update(args.steps)          # returns after the stream has drained
     t1 = time.perf_counter()

@@ -70,7 +70,7 @@ struct dsa_chain {
   bool mat_ok = false;
   double mat_m0 = 0, mat_macc = 0;
   // per-launch timing (measurement mode): one event pair per pass
-  bool profiling = false;
+  int profiling = 0;  // 0 off, 1 an event pair per pass launch, 2 one pair per dsa_chain_sweep call
   std::vector<hipEvent_t> pev;
   double kern_ms = 0;
   uint64_t kern_launches = 0;
@@ -508,7 +508,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.cmargin = (mode == dsa::kClsMatrix ? c->mat_m0 : mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
   A.cmargin_acc = c->mat_macc * c->margin_scale;
   const size_t np = c->S.passes.size();
-  if (c->profiling && c->pev.size() < 2 * np) {
+  if (c->profiling == 1 && c->pev.size() < 2 * np) {
     for (size_t q = c->pev.size(); q < 2 * np; ++q) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
@@ -525,12 +525,12 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
       A.sites = c->d_sites + ps.offset;
       A.count = ps.count;
       A.use_cls = mode;
-      if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q], c->stream));
+      if (c->profiling == 1) HIP_TRY(hipEventRecord(c->pev[2 * q], c->stream));
       HIP_TRY(dsa::launch_sweep(A, c->max_blocks, c->stream));
-      if (c->profiling) HIP_TRY(hipEventRecord(c->pev[2 * q + 1], c->stream));
+      if (c->profiling == 1) HIP_TRY(hipEventRecord(c->pev[2 * q + 1], c->stream));
       c->hit += (uint64_t)ps.count * dsa::kNHit;
     }
-    if (c->profiling) {
+    if (c->profiling == 1) {
       HIP_TRY(hipEventSynchronize(c->pev[2 * np - 1]));
       for (size_t q = 0; q < np; ++q) {
         float t = 0;
@@ -546,6 +546,10 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  if (c->profiling == 2) {
+    c->kern_ms += ms;
+    c->kern_launches += (uint64_t)n * np;
+  }
   if (n) c->lines_valid = false;
   return 0;
 }
@@ -847,7 +851,8 @@ int dsa_chain_last_sweep_ms(dsa_chain *c, double *ms) {
 
 int dsa_chain_set_profiling(dsa_chain *c, int32_t on) {
   if (!c) return fail(DSA_E_ARG, "dsa_chain_set_profiling: NULL chain");
-  c->profiling = on != 0;
+  if (on < 0 || on > 2) return fail(DSA_E_ARG, "dsa_chain_set_profiling: mode must be 0, 1 or 2");
+  c->profiling = on;
   c->kern_ms = 0;
   c->kern_launches = 0;
   return 0;

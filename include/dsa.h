@@ -282,11 +282,14 @@ int dsa_llr_serve(dsa_llr *L, const dsa_group *const *groups, const dsa_llr_para
                   uint32_t nreq, uint32_t chunk, double *admit_s, double *done_s, dsa_llr_status *status);
 
 /* Measurement helpers (no counterpart in the reference, which times with
- * gettimeofday around update(), timer.cpp:11-27).  With profiling on, every
- * pass launch is bracketed by HIP events on the chain's stream and
- * dsa_chain_kernel_time returns the summed kernel milliseconds and launch
- * count since profiling was switched on. */
-int dsa_chain_set_profiling(dsa_chain *c, int32_t on);
+ * gettimeofday around update(), timer.cpp:11-27).  Mode 1: every pass launch
+ * is bracketed by HIP events on the chain's stream (each event record also
+ * orders a cache release between passes, which costs the sweep ~9 %); mode
+ * 2: one event pair around each dsa_chain_sweep call, so the time per launch
+ * includes the gaps between launches.  dsa_chain_kernel_time returns the
+ * summed milliseconds and the pass launches they cover since the mode was
+ * set; mode 0 switches it off. */
+int dsa_chain_set_profiling(dsa_chain *c, int32_t mode);
 int dsa_chain_kernel_time(dsa_chain *c, double *ms, uint64_t *launches);
 /* Random uint16 gathers from the group's P x P table, `blocks` x 256 lanes,
  * 24 gathers in flight per lane, `rounds` rounds: the ceiling the sweep's
