@@ -475,7 +475,7 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
 // sum_k w_k rho(s_k) (w = beta1 for plaquettes, beta2 for rectangles), so
 // with M = S rho(b) a proposal b -> b p changes it by
 // dS = Re Tr M - Re Tr(rho(p) M), and an accepted p takes M to M rho(p).  S is
-// built in FP32 from the links' matrices in LDS, 66 complex 3x3 products per
+// built in FP32 from the links' matrices in LDS, 84 complex 3x3 products per
 // link instead of 85 table gathers from L2; dS then carries a rounding error
 // below the host's bound (margin0, plus margin_acc per accepted hit), and a
 // hit whose decision that error could change is decided from element values

@@ -54,7 +54,7 @@ void k_sweep_cls(SweepArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   // class values at a fixed LDS address: the narrow table's bytes index them
   __shared__ int32_t s_rcq[64];
-  const uint32_t P = A.T.P, np = A.T.npool, nc = A.T.ncls;
+  const uint32_t P = A.T.P, np = A.T.npool;
   uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem);
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(reinterpret_cast<unsigned char *>(s_inv) + pad16(2 * (size_t)P));
   unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
@@ -79,7 +79,6 @@ void k_sweep_cls(SweepArgs A) {
     ClsNarrowStage g;
     g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
   }
-  (void)nc;
   __syncthreads();
 
   const TabView T{A.T.mult, s_inv, P};
@@ -168,7 +167,7 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
   __shared__ int32_t s_rcq[64];
   __shared__ double s_u2[kNHit * NL];
   __shared__ uint16_t s_jp[kNHit * NL];
-  const uint32_t P = A.T.P, np = A.T.npool, nc = A.T.ncls;
+  const uint32_t P = A.T.P, np = A.T.npool;
   uint16_t *s_inv = reinterpret_cast<uint16_t *>(smem);
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(smem + pad16(2 * (size_t)P));
   unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
@@ -177,7 +176,6 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
     ClsNarrowStage g;
     g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
   }
-  (void)nc;
   const uint32_t first = blockIdx.x * NL;
   const uint32_t n = A.count - first < (uint32_t)NL ? A.count - first : (uint32_t)NL;
   const double npd = (double)np;
