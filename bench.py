@@ -17,8 +17,8 @@ Prints one JSON line.  Besides the contract fields it carries
                 per clock); and against a random-gather probe (table gathers
                 per link-update: 608 for the table kernel; 85 + 25 per
                 accepted hit + 98 per exact re-decision for the class
-                kernel; the probe's lanes gather within one table row per
-                round, so part of its rate is served by L1);
+                kernel; the probe's gathers each hit a random table row,
+                as the sweep's do);
   cpu_baseline  the reference binary (oracle/_ref/dym-mod-metro, built from
                 the reference's sources) timed on this host's cores for two
                 sweeps of the same workload, by its own TM_END: update line.

@@ -1,8 +1,9 @@
 // probe.hip -- measurement kernels for the roofline of the sweep.
 //
 // k_gather_probe: random uint16 gathers from a P x P table, GPL independent
-// gathers in flight per lane (the sweep keeps 24 in flight per hit), giving the
-// random-gather ceiling the sweep is bound by.
+// gathers in flight per lane, each in a different (random) row, so that each
+// one is a separate line as the sweep's staple products are: the measured
+// random-gather ceiling (L2 requests) the sweep is bound by.
 // k_copy_probe: a plain device copy of known size, the scale for the
 // FETCH_SIZE / WRITE_SIZE hardware counters.
 #include <hip/hip_runtime.h>
@@ -21,14 +22,14 @@ __global__ __launch_bounds__(256) void k_gather_probe(const uint16_t *__restrict
   const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
 #pragma unroll
   for (int j = 0; j < GPL; ++j) st[j] = (lane * 2654435761u + (uint32_t)j * 40503u) % P;
-  uint32_t row = (lane * 7919u) % P;
+  uint32_t col = (lane * 7919u) % P;
   uint32_t acc = 0;
   for (uint32_t r = 0; r < rounds; ++r) {
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < GPL; ++j) s += table[row * P + st[j]];
+    for (int j = 0; j < GPL; ++j) s += table[st[j] * P + col];
     acc ^= s;
-    row = s % P;  // next row depends on this round, like the next proposal
+    col = s % P;  // the next column depends on this round, like the next proposal
   }
   sink[lane] = acc;  // keeps every gather live
 }

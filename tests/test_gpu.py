@@ -242,6 +242,28 @@ def test_mid_size_passes_class_equals_table(dsa, dgroups, Nt, Nx):
     assert out[0][1] == out[1][1]
 
 
+def test_profiling_modes_count_launches(dsa, dgroups):
+    """Mode 1 times every pass launch, mode 2 each sweep call as a whole;
+    both count the pass launches they cover, and the links do not depend on
+    the mode."""
+    import ctypes as C
+    say("case profiling modes")
+    g = dgroups("mys1080-v4")
+    links = []
+    for mode in (1, 2):
+        with dsa.Chain(g, 4, 8, 8, 0.0, 3.0, -0.25, 4) as c:
+            npass, _ = c.schedule_info()
+            assert dsa.lib().dsa_chain_set_profiling(c._h, mode) == 0
+            c.update(2)
+            c.update(1)
+            ms, nl = C.c_double(), C.c_uint64()
+            assert dsa.lib().dsa_chain_kernel_time(c._h, C.byref(ms), C.byref(nl)) == 0
+            assert nl.value == 3 * npass and ms.value > 0
+            links.append(c.a)
+            assert dsa.lib().dsa_chain_set_profiling(c._h, 3) != 0
+    assert np.array_equal(links[0], links[1])
+
+
 def test_resume_from_checkpoint_is_bit_exact(dsa, dgroups):
     say("case checkpoint/resume")
     g = dgroups("mys1080-v4")
