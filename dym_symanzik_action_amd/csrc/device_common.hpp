@@ -399,33 +399,50 @@ struct PreDraws {
   uint32_t stride, l;
 };
 
-template <int D, class L>
-__device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const ClsCtx &K, const Geom &G,
-                                           const TabView &T, int mu, uint32_t i, uint32_t &nfall,
-                                           const PreDraws *pre = nullptr) {
-  constexpr int NS = 2 * (D - 1), NR = 6 * (D - 1), NL = NS + NR;
-  const Site<D> s = decode<D>(G, i);
-  const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
-  uint32_t b = C.links[lp];
-  const uint32_t b0 = b;
-  uint32_t t[NL];  // t_k = s_k b (plaquettes first)
-  build_t<D>(C.links, G, T, s, mu, b, t);
-  // class sums of the plaquette (k < NS) and rectangle terms through `row`
-  auto sums = [&](typename L::Row row, int32_t &r, int32_t &m) {
-    int32_t a = 0, c = 0;
-#pragma unroll
-    for (int k = 0; k < NL; ++k) {
-      const int32_t v = Lk.val(row, t[k]);
-      if (k < NS) a += v; else c += v;
-    }
-    r = a;
-    m = c;
-  };
+// One link's state between the two halves of the class update: its
+// staple-times-link products t_k = s_k b (plaquettes first) with their class
+// sums, built by cls_prepare; its hits, made by cls_hits.
+template <int D>
+struct ClsLink {
+  static constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
+  Site<D> s;
+  size_t lp;
+  uint32_t i, b, b0, ms;
+  uint32_t t[NL];
   int32_t r1, m1;
-  sums(Lk.row0(), r1, m1);
-  uint32_t ms = 0;
-  if (C.rng == kRngMinstd) ms = C.mstate[i];
-  const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
+};
+
+// Class sums of the plaquette (k < NS) and rectangle terms of t through `row`.
+template <int D, class L>
+__device__ inline void cls_sums(const L &Lk, typename L::Row row, const uint32_t *t, int32_t &r, int32_t &m) {
+  constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
+  int32_t a = 0, c = 0;
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int32_t v = Lk.val(row, t[k]);
+    if (k < NS) a += v; else c += v;
+  }
+  r = a;
+  m = c;
+}
+
+template <int D, class L>
+__device__ inline void cls_prepare(ClsLink<D> &S, const LinkCtx &C, const L &Lk, const Geom &G, const TabView &T,
+                                   int mu, uint32_t i) {
+  S.i = i;
+  S.s = decode<D>(G, i);
+  S.lp = (size_t)mu * G.V + lpos(G, i, S.s.par);
+  S.b = S.b0 = C.links[S.lp];
+  build_t<D>(C.links, G, T, S.s, mu, S.b, S.t);
+  cls_sums<D>(Lk, Lk.row0(), S.t, S.r1, S.m1);
+  S.ms = C.rng == kRngMinstd ? C.mstate[i] : 0u;
+}
+
+template <int D, class L>
+__device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk, const ClsCtx &K, const TabView &T,
+                                    int mu, uint32_t &nfall, const PreDraws *pre) {
+  constexpr int NL = ClsLink<D>::NL;
+  const uint32_t link = S.i * (uint32_t)D + (uint32_t)mu;
   const double b1q = C.beta1 * L::kScale, b2q = C.beta2 * L::kScale, margin = K.margin;
   uint32_t acc = 0;
   U4 w{0, 0, 0, 0};
@@ -440,32 +457,41 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
       jp = hit_pool_index((h & 1) ? w.z : w.x, C.npool);
       u2 = hit_uniform((h & 1) ? w.w : w.y);
     } else {
-      jp = (uint32_t)(C.npool_d * minstd_canonical(ms));
-      u2 = minstd_canonical(ms);
+      jp = (uint32_t)(C.npool_d * minstd_canonical(S.ms));
+      u2 = minstd_canonical(S.ms);
     }
     int32_t r1n, m1n;
-    sums(Lk.row(jp, T.P), r1n, m1n);
+    cls_sums<D>(Lk, Lk.row(jp, T.P), S.t, r1n, m1n);
     // class-valued change of action; the exact one lies within M of it
-    const double dc = b1q * ((double)r1n - (double)r1) + b2q * ((double)m1n - (double)m1);
+    const double dc = b1q * ((double)r1n - (double)S.r1) + b2q * ((double)m1n - (double)S.m1);
     const double M = margin + 0x1p-50 * fabs(dc);
     int dec = metro_certified(dc, M, u2);
     const uint32_t p = C.s_pool[jp];
     if (dec < 0) {
-      dec = exact_hit<D>(C, K.retr, T, b, p, t, u2);
+      dec = exact_hit<D>(C, K.retr, T, S.b, p, S.t, u2);
       ++nfall;
     }
     if (dec) {
-      b = T.M(b, p);
+      S.b = T.M(S.b, p);
 #pragma unroll
-      for (int k = 0; k < NL; ++k) t[k] = T.M(t[k], p);
-      r1 = r1n;
-      m1 = m1n;
+      for (int k = 0; k < NL; ++k) S.t[k] = T.M(S.t[k], p);
+      S.r1 = r1n;
+      S.m1 = m1n;
       ++acc;
     }
   }
-  if (b != b0) C.links[lp] = (uint16_t)b;
-  if (C.rng == kRngMinstd) C.mstate[i] = ms;
+  if (S.b != S.b0) C.links[S.lp] = (uint16_t)S.b;
+  if (C.rng == kRngMinstd) C.mstate[S.i] = S.ms;
   return acc;
+}
+
+template <int D, class L>
+__device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const ClsCtx &K, const Geom &G,
+                                           const TabView &T, int mu, uint32_t i, uint32_t &nfall,
+                                           const PreDraws *pre = nullptr) {
+  ClsLink<D> S;
+  cls_prepare<D>(S, C, Lk, G, T, mu, i);
+  return cls_hits<D>(S, C, Lk, K, T, mu, nfall, pre);
 }
 
 // Matrix form of the hit loop.  rho(g) is the SU(3) matrix the group file
