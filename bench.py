@@ -274,9 +274,11 @@ def main():
     if form == "table":
         gathers_per_unit = float(GATHERS_TABLE)
     elif form == "matrix":
-        # the new link of an accepted hit; an exact re-decision builds the 24
-        # staple-times-link products (85) and evaluates them (98)
-        gathers_per_unit = ((acc1 - acc0) + (GATHERS_CLASS + GATHERS_CLASS_EXACT) * (ex1 - ex0)) / (args.steps * LINKS)
+        # the 24 staple-times-link products (85), the new link of an accepted
+        # hit; an exact re-decision builds the products again (85) and
+        # evaluates them (98)
+        gathers_per_unit = GATHERS_CLASS + ((acc1 - acc0) + (GATHERS_CLASS + GATHERS_CLASS_EXACT) *
+                                            (ex1 - ex0)) / (args.steps * LINKS)
     else:
         gathers_per_unit = GATHERS_CLASS + (GATHERS_CLASS_ACCEPT * (acc1 - acc0) +
                                             GATHERS_CLASS_EXACT * (ex1 - ex0)) / (args.steps * LINKS)

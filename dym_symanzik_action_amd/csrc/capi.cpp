@@ -133,22 +133,21 @@ bool dsa::su3_check(const Group &g, std::vector<float> &mat32, double &dev) {
   return true;
 }
 
-// The matrix kernel's bound.  With u = 2^-24, every FP32 matrix entry
-// rounded from the file (|e| <= sqrt(3) u in Frobenius norm), every product
-// of 3x3 complex matrices a chain of six FMAs per real part (error <=
-// sqrt(2) gamma_6 |A||B|, Frobenius <= 25.5 u ||A||_F), a staple a chain of
-// at most 5 products, accumulated with weights w_k into S and multiplied by
-// rho(b): ||M_computed - M||_F <= ~170 u W with W = sum |w_k| =
-// 2(D-1)|beta1| + 6(D-1)|beta2|.  dS = Re Tr M - Re Tr(rho(p) M) then errs
-// by at most 2 sqrt(3) ||dM||_F + 54 u W (the 18-FMA trace) + ~10 u W (the
-// other trace, rho(p)'s rounding, the weights' rounding): ~660 u W.  Each
-// accepted hit's product M rho(p) adds ~100 u W.  The bound below takes
-// 2048 u W and 256 u W (safety >= 2.5), plus the file's own deviation from
-// the tables (su3_check) and the rounding of the reference's double sums.
+// The matrix kernel's bound.  With u = 2^-24: M = sum_k w_k rho(t_k) is
+// accumulated with 24 FMAs per entry from matrices whose entries (|.| <= 1,
+// unitary) are rounded from the file, with weights rounded to float, so each
+// real and imaginary entry errs by <= 26 u W, W = sum |w_k| =
+// 2(D-1)|beta1| + 6(D-1)|beta2|.  Re Tr M (three adds) then errs by <= 84 u W;
+// Re Tr(rho(p) M), an 18-FMA chain over terms whose magnitudes sum to
+// <= 3 sqrt(6) W, by <= 3 sqrt(6) 26 u W (M's error) + 132 u W (the chain)
+// + 8 u W (rho(p)'s rounding): dS errs by < 420 u W.  Each accepted hit's
+// product M rho(p) (six-FMA chains per entry) adds < 100 u W.  The bound below
+// takes 1024 u W and 256 u W (safety >= 2.4), plus the file's own deviation
+// from the tables (su3_check) and the rounding of the reference's double sums.
 void dsa::mat_margin(int D, double beta0, double beta1, double beta2, double dev, double &m0, double &macc) {
   const double u = 0x1p-24, ud = 0x1p-53;
   const double W = 2.0 * (D - 1) * std::fabs(beta1) + 6.0 * (D - 1) * std::fabs(beta2);
-  m0 = W * (2048.0 * u + 64.0 * dev + 1e-12) + 16.0 * ud * (std::fabs(beta0) + 3.0 * W);
+  m0 = W * (1024.0 * u + 64.0 * dev + 1e-12) + 16.0 * ud * (std::fabs(beta0) + 3.0 * W);
   macc = W * 256.0 * u;
 }
 

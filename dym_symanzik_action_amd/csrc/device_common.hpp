@@ -496,16 +496,18 @@ __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const 
 
 // Matrix form of the hit loop.  rho(g) is the SU(3) matrix the group file
 // appends to element g (ReTr(g) = -Re Tr rho(g), rho(a) rho(b) = rho(ab);
-// both to ~1e-15, checked on the host).  The action of link b against its
-// staples is beta0 - Re Tr(rho(b) S) with the weighted staple sum S =
-// sum_k w_k rho(s_k) (w = beta1 for plaquettes, beta2 for rectangles), so
-// with M = S rho(b) a proposal b -> b p changes it by
-// dS = Re Tr M - Re Tr(rho(p) M), and an accepted p takes M to M rho(p).  S is
-// built in FP32 from the links' matrices in LDS, 84 complex 3x3 products per
-// link instead of 85 table gathers from L2; dS then carries a rounding error
-// below the host's bound (margin0, plus margin_acc per accepted hit), and a
-// hit whose decision that error could change is decided from element values
-// by exact_hit: the links and counters are update_link's.
+// both to ~1e-15, checked on the host).  The reference's action of link b is
+// beta0 + sum_k w_k ReTr(b s_k) (w = beta1 for plaquettes, beta2 for
+// rectangles) = beta0 - Re Tr M with M = sum_k w_k rho(t_k), t_k = s_k b (a
+// trace is cyclic), so a proposal b -> b p changes it by
+// dS = Re Tr M - Re Tr(rho(p) M), and an accepted p takes M to M rho(p).  The
+// t_k come from table gathers exactly as in the class kernel, M from their
+// matrices in LDS (24 FP32 axpys); each hit then costs one matrix read and 18
+// FMAs instead of 24 class lookups, and an accepted hit one gather and a 3x3
+// product instead of 25 gathers.  dS carries a rounding error below the
+// host's bound (margin0, plus margin_acc per accepted hit), and a hit whose
+// decision that error could change is decided from element values by
+// exact_hit: the links and counters are update_link's.
 
 struct Cm3 {
   float re[9], im[9];  // row-major entries
@@ -518,31 +520,31 @@ __device__ inline Cm3 mat_of(const float *s_mat, uint32_t e) {
   return Cm3{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x}, {c.y, c.z, c.w, d.x, d.y, d.z, d.w, f.x, f.y}};
 }
 
-// op(A) op(B), op the identity or the conjugate transpose (DA, DB); each
-// entry's real and imaginary part a chain of six FMAs.
-template <bool DA, bool DB>
-__device__ inline Cm3 mm(const Cm3 &A, const Cm3 &B) {
-  Cm3 C;
+// A <- A B, one row at a time (six temporaries instead of a third matrix);
+// each entry's real and imaginary part a chain of six FMAs.
+__device__ inline void mul_right(Cm3 &A, const Cm3 &B) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < 3; ++i) {
+    float r[3], m[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      float r = 0.f, m = 0.f;
+      r[j] = 0.f;
+      m[j] = 0.f;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const float ar = DA ? A.re[3 * k + i] : A.re[3 * i + k];
-        const float ai = DA ? -A.im[3 * k + i] : A.im[3 * i + k];
-        const float br = DB ? B.re[3 * j + k] : B.re[3 * k + j];
-        const float bi = DB ? -B.im[3 * j + k] : B.im[3 * k + j];
-        r = fmaf(ar, br, r);
-        r = fmaf(-ai, bi, r);
-        m = fmaf(ar, bi, m);
-        m = fmaf(ai, br, m);
+        const float ar = A.re[3 * i + k], ai = A.im[3 * i + k], br = B.re[3 * k + j], bi = B.im[3 * k + j];
+        r[j] = fmaf(ar, br, r[j]);
+        r[j] = fmaf(-ai, bi, r[j]);
+        m[j] = fmaf(ar, bi, m[j]);
+        m[j] = fmaf(ai, br, m[j]);
       }
-      C.re[3 * i + j] = r;
-      C.im[3 * i + j] = m;
     }
-  return C;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      A.re[3 * i + j] = r[j];
+      A.im[3 * i + j] = m[j];
+    }
+  }
 }
 
 // Y += a X
@@ -585,97 +587,6 @@ __device__ inline float retr_prod(const Cm3 &A, const Cm3 &B) {
   return t;
 }
 
-// The weighted staple sum S of link (x, mu): each staple of build_staples
-// as a chain of products accumulated into S: 14 products per side and
-// transverse direction (84 per link at D = 4).  Scheduling
-// barriers after each product keep the compiler from hoisting every
-// matrix's LDS load ahead (18 registers each): a handful of matrices is live
-// at a time and the kernel fits 128 registers.
-template <int D>
-__device__ inline Cm3 staple_sum(const uint16_t *links, const Geom &G, const Site<D> &s, int mu, const float *s_mat,
-                                 float w1, float w2) {
-  // the element of U_dir(x + a mu + b nu)
-  auto E = [&](int dir, int a, int nu, int bb) { return U<D>(links, G, s, dir, mu, a, nu, bb); };
-  // T <- T op(rho(e)), one product at a time
-  auto step = [&](Cm3 &T, uint32_t e, auto dag) {
-    T = mm<false, decltype(dag)::value>(T, mat_of(s_mat, e));
-    __builtin_amdgcn_sched_barrier(0);
-  };
-  using N = std::false_type;
-  using H = std::true_type;
-  const uint32_t eup = E(mu, 1, mu, 0), eum = E(mu, -1, mu, 0);
-  Cm3 S{};
-#pragma unroll 1
-  for (int j = 0; j < D - 1; ++j) {
-    const int nu = j + (j >= mu ? 1 : 0);
-    // this direction's 20 links, loaded together ahead of the products and
-    // held two 16-bit indices per register
-    const uint32_t q0 = E(nu, 1, nu, 0) | E(nu, 0, nu, 0) << 16, q1 = E(nu, 1, nu, -1) | E(nu, 0, nu, -1) << 16;
-    const uint32_t q2 = E(nu, 1, nu, 1) | E(nu, 0, nu, 1) << 16, q3 = E(nu, 2, nu, 0) | E(nu, -1, nu, 0) << 16;
-    const uint32_t q4 = E(nu, 1, nu, -2) | E(nu, 0, nu, -2) << 16, q5 = E(nu, 2, nu, -1) | E(nu, -1, nu, -1) << 16;
-    const uint32_t q6 = E(mu, 0, nu, 1) | E(mu, 0, nu, -1) << 16, q7 = E(mu, 0, nu, 2) | E(mu, 1, nu, 1) << 16;
-    const uint32_t q8 = E(mu, -1, nu, 1) | E(mu, 0, nu, -2) << 16, q9 = E(mu, 1, nu, -1) | E(mu, -1, nu, -1) << 16;
-    __builtin_amdgcn_sched_barrier(0);
-    const uint32_t n10 = q0 & 0xffffu, n00 = q0 >> 16, n1m = q1 & 0xffffu, n0m = q1 >> 16;
-    const uint32_t n11 = q2 & 0xffffu, n01 = q2 >> 16, n20 = q3 & 0xffffu, nm0 = q3 >> 16;
-    const uint32_t n1M = q4 & 0xffffu, n0M = q4 >> 16, n2m = q5 & 0xffffu, nmm = q5 >> 16;
-    const uint32_t m01 = q6 & 0xffffu, m0m = q6 >> 16, m02 = q7 & 0xffffu, m11 = q7 >> 16;
-    const uint32_t mm1 = q8 & 0xffffu, m0M = q8 >> 16, m1m = q9 & 0xffffu, mmm = q9 >> 16;
-    Cm3 T;
-    // plaquette up, k2
-    T = mat_of(s_mat, n10);
-    step(T, m01, H{});
-    step(T, n00, H{});
-    axpy(S, w1, T);
-    T = mat_of(s_mat, n10);
-    step(T, m01, H{});
-    step(T, mm1, H{});
-    step(T, nm0, H{});
-    step(T, eum, N{});
-    axpy(S, w2, T);
-    // k0
-    T = mat_of(s_mat, n10);
-    step(T, n11, N{});
-    step(T, m02, H{});
-    step(T, n01, H{});
-    step(T, n00, H{});
-    axpy(S, w2, T);
-    // k1
-    T = mat_of(s_mat, eup);
-    step(T, n20, N{});
-    step(T, m11, H{});
-    step(T, m01, H{});
-    step(T, n00, H{});
-    axpy(S, w2, T);
-    // plaquette down, k5
-    T = mm<true, true>(mat_of(s_mat, n1m), mat_of(s_mat, m0m));
-    __builtin_amdgcn_sched_barrier(0);
-    step(T, n0m, N{});
-    axpy(S, w1, T);
-    T = mm<true, true>(mat_of(s_mat, n1m), mat_of(s_mat, m0m));
-    __builtin_amdgcn_sched_barrier(0);
-    step(T, mmm, H{});
-    step(T, nmm, N{});
-    step(T, eum, N{});
-    axpy(S, w2, T);
-    // k3
-    T = mm<true, true>(mat_of(s_mat, n1m), mat_of(s_mat, n1M));
-    __builtin_amdgcn_sched_barrier(0);
-    step(T, m0M, H{});
-    step(T, n0M, N{});
-    step(T, n0m, N{});
-    axpy(S, w2, T);
-    // k4
-    T = mat_of(s_mat, eup);
-    step(T, n2m, H{});
-    step(T, m1m, H{});
-    step(T, m0m, H{});
-    step(T, n0m, N{});
-    axpy(S, w2, T);
-  }
-  return S;
-}
-
 struct MatCtx {
   const float *s_mat;         // LDS, P x 20 floats
   const double *retr;         // global, element ReTr for exact decisions
@@ -685,7 +596,7 @@ struct MatCtx {
 template <int D>
 __device__ inline uint32_t update_link_mat(const LinkCtx &C, const MatCtx &K, const Geom &G, const TabView &T, int mu,
                                            uint32_t i, uint32_t &nfall) {
-  constexpr int NL = 8 * (D - 1);
+  constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
   const Site<D> s = decode<D>(G, i);
   const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
   uint32_t b = C.links[lp];
@@ -706,47 +617,56 @@ __device__ inline uint32_t update_link_mat(const LinkCtx &C, const MatCtx &K, co
     }
   };
   uint32_t acc = 0;
-  int h = 0;
-  uint32_t jp = 0;
-  double u2 = 0;
   {
-    Cm3 M = mm<false, false>(staple_sum<D>(C.links, G, s, mu, K.s_mat, (float)C.beta1, (float)C.beta2),
-                             mat_of(K.s_mat, b));
+    Cm3 M;
+    {
+      uint32_t t[NL];
+      build_t<D>(C.links, G, T, s, mu, b, t);
+      // every gather issued before the first matrix is read; then one
+      // matrix (20 registers) at a time
+      __builtin_amdgcn_sched_barrier(0);
+      const float w1 = (float)C.beta1, w2 = (float)C.beta2;
+      M = scaled(w1, mat_of(K.s_mat, t[0]));
+#pragma unroll
+      for (int k = 1; k < NL; ++k) {
+        __builtin_amdgcn_sched_barrier(0);
+        axpy(M, k < NS ? w1 : w2, mat_of(K.s_mat, t[k]));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
     float v0 = M.re[0] + M.re[4] + M.re[8];
     double margin = K.margin0;
-    for (; h < kNHit; ++h) {
+    for (int h = 0; h < kNHit; ++h) {
+      uint32_t jp;
+      double u2;
       draw(h, jp, u2);
       const uint32_t p = C.s_pool[jp];
-      const Cm3 R = mat_of(K.s_mat, p);
-      // dS = Re Tr M - Re Tr(rho(p) M); the difference of two floats is exact in double
-      const double dc = (double)v0 - (double)retr_prod(R, M);
-      const int dec = metro_certified(dc, margin + 0x1p-50 * fabs(dc), u2);
-      if (dec < 0) break;  // this hit and the rest from element values, below
+      // dS = Re Tr M - Re Tr(rho(p) M); the difference of two floats is exact
+      // in double.  rho(p) is read again for an accepted hit rather than
+      // kept live across the element-value path.
+      const double dc = (double)v0 - (double)retr_prod(mat_of(K.s_mat, p), M);
+      int dec = metro_certified(dc, margin + 0x1p-50 * fabs(dc), u2);
+      if (dec < 0) {
+        // rare: this hit from element values (t_k = s_k b as the class
+        // kernel builds them); M goes on estimating the later hits.  The
+        // site is decoded again from an opaque copy of i: otherwise the
+        // compiler keeps the first build_t's link addresses live through
+        // the hit loop for this path (spilling them on the common one).
+        uint32_t i2 = i;
+        asm volatile("" : "+v"(i2));
+        const Site<D> s2 = decode<D>(G, i2);
+        uint32_t t[NL];
+        build_t<D>(C.links, G, T, s2, mu, b, t);
+        dec = exact_hit<D>(C, K.retr, T, b, p, t, u2);
+        ++nfall;
+      }
       if (dec) {
         b = T.M(b, p);
-        M = mm<false, false>(M, R);
+        mul_right(M, mat_of(K.s_mat, p));
         v0 = M.re[0] + M.re[4] + M.re[8];
         margin += K.margin_acc;
         ++acc;
       }
-    }
-  }
-  if (h < kNHit) {
-    // rare: from the undecided hit on, element values (t_k = s_k b as the
-    // class kernel builds them), hit h's draws already taken
-    uint32_t t[NL];
-    build_t<D>(C.links, G, T, s, mu, b, t);
-    for (;;) {
-      const uint32_t p = C.s_pool[jp];
-      ++nfall;
-      if (exact_hit<D>(C, K.retr, T, b, p, t, u2)) {
-        b = T.M(b, p);
-#pragma unroll
-        for (int k = 0; k < NL; ++k) t[k] = T.M(t[k], p);
-        ++acc;
-      }
-      if (++h >= kNHit) break;
-      draw(h, jp, u2);
     }
   }
   if (b != b0) C.links[lp] = (uint16_t)b;

@@ -59,8 +59,8 @@ void class_margin_terms(const Group &g, int D, double q, double &e1, double &e2,
 // rho(a) rho(b) - rho(ab) over all pairs.  False when the file has none.
 bool su3_check(const Group &g, std::vector<float> &mat32, double &dev);
 // Bound on |dS_exact - dS_matrix| for the matrix kernel's first hit, and its
-// growth per accepted hit (the FP32 rounding of the staple products, their
-// weighted sum and the traces; see su3 notes in capi.cpp).
+// growth per accepted hit (the FP32 rounding of the matrices, their weighted
+// sum, the traces and the products M rho(p)).
 void mat_margin(int D, double beta0, double beta1, double beta2, double dev, double &m0, double &macc);
 // Shape and option checks shared by chains and batches.
 int check_params(const dsa_chain_params *p);

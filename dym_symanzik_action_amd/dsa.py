@@ -266,7 +266,7 @@ class Chain:
     def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
         """Sweep kernel: "table" (element ReTr via L2 gathers), "class" (class
         values in LDS, exact re-decision near the bound), "class-narrow" (the
-        byte class-index form), "matrix" (FP32 products of the group's SU(3)
+        byte class-index form), "matrix" (trial actions as FP32 traces of the group's SU(3)
         matrices in LDS, exact re-decision near the rounding bound; groups
         whose file carries the matrices), or "auto"; all give identical
         links.  A huge margin_scale re-decides every hit exactly."""

@@ -146,9 +146,10 @@ int dsa_chain_save_kentucky(dsa_chain *c, const char *prefix, int32_t num, char 
  * device's LDS (it is faster at every lattice size measured, 8^4 to
  * 32^3 x 64), in the byte-index form where that fits, else as CLASS.
  * MATRIX (groups whose file appends SU(3) matrices, e.g. S(1080)): the
- * staple sum and the trial actions as FP32 3x3 complex products of those
- * matrices in LDS instead of table gathers; a hit whose decision the
- * rounding bound could change is decided again from element values.
+ * staple-times-link products by table gathers, their weighted sum as an FP32
+ * 3x3 complex matrix from those matrices in LDS, and each trial action as one
+ * trace of a product with it; a hit whose decision the rounding bound could
+ * change is decided again from element values.
  * margin_scale (default 1) multiplies the bound
  * that selects those re-decisions: a huge value makes every hit re-decided
  * (a test of that path). */
