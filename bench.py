@@ -11,17 +11,29 @@ Prints one JSON line.  Besides the contract fields it carries
   roofline      the sweep kernel against HBM (66 algorithmic bytes of link
                 traffic per link-update), with the time per launch from a
                 HIP event pair on the chain's stream around the timed sweeps;
-                against the L2 request rate
-                that its table gathers meet (L1->L2 read requests per launch
-                from the committed counters vs one request per L2 channel
-                per clock); and against a random-gather probe (table gathers
+                and, under "limiter", what bounds it: the L2 request rate
+                its table gathers meet (L1->L2 read requests per launch from
+                the committed rocprofv3 counters vs one request per L2
+                channel per clock) and a random-gather probe (table gathers
                 per link-update: 608 for the table kernel; 85 + 25 per
                 accepted hit + 98 per exact re-decision for the class
                 kernel; the probe's gathers each hit a random table row,
                 as the sweep's do);
+  beta_scan     the 32^3 x 64 lattice at beta1 in {2.0, 2.5, 3.0, 3.5} x
+                beta2 in {0, -0.25}, cold and hot (seed 11): link-updates/s,
+                acceptance and exact re-decisions per hit, two sweeps each;
+  batched_vs_serial  the 256-chain phase-diagram scan run as 30 hipGraph-
+                captured cycles of 10 sweeps + measure (300 sweeps), against
+                the same chains swept one at a time;
+  llr_serving   a Poisson stream of LLR window requests served with
+                continuous batching under an HBM budget: p50/p99 latency to a
+                converged a_n, windows/s, and a sample of the requests one
+                at a time;
   cpu_baseline  the reference binary (oracle/_ref/dym-mod-metro, built from
-                the reference's sources) timed on this host's cores for two
-                sweeps of the same workload, by its own TM_END: update line.
+                the reference's sources) timed on this host for three sweeps
+                of the same workload by its own TM_END: update line, on the
+                host threads this process may use (at most 16), with a
+                one-thread run beside it.
 """
 import argparse
 import gzip
@@ -35,6 +47,8 @@ import time
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
+
+from dym_symanzik_action_amd import workloads  # noqa: E402
 
 D, NT, NX = 4, 64, 32
 V = NT * NX ** 3
@@ -73,8 +87,62 @@ def read_profile(form, key):
     return None
 
 
-def cpu_baseline(group_gz, cores):
-    """Two sweeps of the same workload by the reference binary, on `cores` threads."""
+def host_cpu():
+    """CPU model, physical cores, logical CPUs and the CPUs this process may use."""
+    model, phys, logical = None, set(), 0
+    cur = {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                cur[k] = v
+                if k == "model name" and model is None:
+                    model = v
+            elif cur:
+                logical += 1
+                phys.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+        if cur:
+            logical += 1
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "physical_cores": len(phys) or None, "logical_cpus": logical or None,
+            "usable_cpus": usable}
+
+
+def reference_sweeps(gtxt, td, shape, nsw, threads, timeout_s=600):
+    """Seconds the reference binary spends in `nsw` sweeps of the benchmark's
+    couplings and seed on `shape` = (Nt, Nx), by its own TM_END: update line
+    (the process is stopped there: its Nx x Nt Wilson grid measurement would
+    take hours at 32^3 x 64)."""
+    ref = os.path.join(REPO, "oracle", "_ref", "dym-mod-metro")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    nt, nx = shape
+    cmd = [ref, gtxt, str(D), str(nt), str(nx), *map(repr, BETA), str(SEED), str(nsw), "1", "0"]
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, env=env, cwd=td)
+    t = None
+    deadline = time.time() + timeout_s
+    for line in p.stdout:
+        if line.startswith("TM_END:") and "update" in line:
+            t = float(line.split("done in")[1].split()[0])
+            break
+        if time.time() > deadline:
+            break
+    p.kill()
+    p.wait()
+    return t
+
+
+def cpu_baseline(group_gz, threads, nsw=3, nsw1=3, shape1=(32, 16)):
+    """The reference binary on this host: `nsw` sweeps of the benchmark
+    workload on `threads` OpenMP threads, and `nsw1` sweeps on one thread of
+    the same couplings on the 16^3 x 32 lattice (one thread takes ~40 s per
+    32^3 x 64 sweep)."""
     ref = os.path.join(REPO, "oracle", "_ref", "dym-mod-metro")
     if not os.path.exists(ref):
         return None
@@ -83,131 +151,129 @@ def cpu_baseline(group_gz, cores):
         gtxt = os.path.join(td, "mys1080-v4")
         with gzip.open(group_gz, "rb") as fi, open(gtxt, "wb") as fo:
             shutil.copyfileobj(fi, fo)
-        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-        nsw = 2
-        cmd = [ref, gtxt, str(D), str(NT), str(NX), *map(repr, BETA), str(SEED), str(nsw), "1", "0"]
-        p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, env=env, cwd=td)
-        t = None
-        deadline = time.time() + 300
-        for line in p.stdout:
-            if line.startswith("TM_END:") and "update" in line:
-                t = float(line.split("done in")[1].split()[0])
-                break
-            if time.time() > deadline:
-                break
-        p.kill()
-        p.wait()
+        t = reference_sweeps(gtxt, td, (NT, NX), nsw, threads)
         if t is None:
             return None
-        return {"value": nsw * LINKS / t, "unit": "link-updates/s", "cores": cores, "kind": "reference",
+        links1 = D * shape1[0] * shape1[1] ** (D - 1)
+        t1 = reference_sweeps(gtxt, td, shape1, nsw1, 1)
+        cpu = host_cpu()
+        return {"value": nsw * LINKS / t, "unit": "link-updates/s", "cores": threads, "kind": "reference",
                 "sample": f"{nsw} sweeps of 32^3x64 S1080 hot seed {SEED} beta={BETA} ({nsw} x 8,388,608 "
-                          "link-updates), "
-                          f"reference dym-mod-metro built from its sources (-O3 -fopenmp), OMP_NUM_THREADS={cores}, "
-                          f"timed by its own TM_END: update line ({t:.2f} s)"}
+                          f"link-updates), reference dym-mod-metro built from its sources (-O3 -fopenmp), "
+                          f"OMP_NUM_THREADS={threads}, timed by its own TM_END: update line ({t:.2f} s)",
+                "host": cpu,
+                "one_thread": {"value": nsw1 * links1 / t1 if t1 else None, "unit": "link-updates/s", "cores": 1,
+                               "sample": f"{nsw1} sweeps of {shape1[1]}^3x{shape1[0]} S1080, same couplings and "
+                                         f"seed, OMP_NUM_THREADS=1 ({t1:.2f} s)" if t1 else "timed out"}}
     finally:
         shutil.rmtree(td, ignore_errors=True)
 
 
-def scan_chains():
-    """The phase-diagram scan workload: 256 chains on a 16 x 16 grid of
-    beta1 in [1, 4] x beta2 in [-0.5, 0.3], shapes round-robin over 8^4, 12^4,
-    16^3 x 8, 8^3 x 16, seeds 100 + i, hot; chains 192..255 use S(216)."""
-    shapes = [(4, 8, 8), (4, 12, 12), (4, 8, 16), (4, 16, 8)]  # (D, Nt, Nx)
-    out = []
-    for i in range(256):
-        b1 = 1.0 + 3.0 * (i // 16) / 15.0
-        b2 = -0.5 + 0.8 * (i % 16) / 15.0
-        out.append(("S216" if i >= 192 else "mys1080-v4", shapes[i % 4], b1, b2, 100 + i))
-    return out
+def beta_scan(dsa, g, sweeps=2):
+    """The 32^3 x 64 lattice over the beta grid of the large-shape workload:
+    beta1 in {2.0, 2.5, 3.0, 3.5} x beta2 in {0, -0.25}, cold and hot (seed
+    11); per point, link-updates/s over `sweeps` sweeps after one untimed
+    sweep, acceptance and exact re-decisions per hit."""
+    rows = []
+    for start in ("cold", "hot"):
+        for b2 in (0.0, -0.25):
+            for b1 in (2.0, 2.5, 3.0, 3.5):
+                with dsa.Chain(g, D, NT, NX, 0.0, b1, b2, SEED, start=start) as c:
+                    c.update(1)
+                    h0, a0 = c.counters()
+                    e0 = c.exact_decisions()
+                    t0 = time.perf_counter()
+                    c.update(sweeps)
+                    dt = time.perf_counter() - t0
+                    h1, a1 = c.counters()
+                    e1 = c.exact_decisions()
+                rows.append({"start": start, "beta1": b1, "beta2": b2, "link_updates_per_s": sweeps * LINKS / dt,
+                             "acceptance": (a1 - a0) / (h1 - h0), "exact_redecisions_per_hit": (e1 - e0) / (h1 - h0)})
+                log(f"bench: beta scan {start} ({b1}, {b2}): {rows[-1]['link_updates_per_s']:.3e} link-updates/s, "
+                    f"acceptance {rows[-1]['acceptance']:.4f}")
+    lu = [r["link_updates_per_s"] for r in rows]
+    return {"workload": f"32^3x64 S(1080), beta1 in {{2.0,2.5,3.0,3.5}} x beta2 in {{0,-0.25}}, cold and hot seed "
+                        f"{SEED}; {sweeps} timed sweeps per point after one untimed", "points": rows,
+            "min_link_updates_per_s": min(lu), "max_link_updates_per_s": max(lu)}
 
 
-def batched_vs_serial(dsa, sweeps):
-    """Link-updates/s of the 256-chain scan swept as one batch, and one chain
-    at a time (each chain's own launches), over the same number of sweeps."""
+def batched_vs_serial(dsa, cycles, sweeps_per_cycle, serial_sweeps):
+    """Link-updates/s of the 256-chain scan as one batch -- `cycles`
+    hipGraph-captured cycles of `sweeps_per_cycle` sweeps + measure -- and
+    one chain at a time (each chain's own sweep launches, `serial_sweeps`
+    sweeps per chain)."""
     groups = {n: dsa.load_group(dsa.group_path(n)) for n in ("mys1080-v4", "S216")}
-    chains = scan_chains()
-    links = sum(s[1] * s[2] ** 3 * s[0] for _, s, *_ in chains)
+    chains = workloads.scan_chains()
+    links = workloads.scan_links(chains)
     b = dsa.Batch()
-    for name, (D, Nt, Nx), b1, b2, seed in chains:
-        b.add(groups[name], D, Nt, Nx, 0.0, b1, b2, seed)
+    for name, (d, nt, nx), b1, b2, seed in chains:
+        b.add(groups[name], d, nt, nx, 0.0, b1, b2, seed)
     b.finalize()  # times 256/512/1024 links per block and keeps the fastest
     task_links, tune_ms = b.task_links()
-    b.sweep(1)
+    b.run_cycles(1, sweeps_per_cycle)  # capture the graph, one untimed cycle
     t0 = time.perf_counter()
-    b.sweep(sweeps)
+    meas = b.run_cycles(cycles, sweeps_per_cycle)
     tb = time.perf_counter() - t0
     b.close()
     objs = []
-    for name, (D, Nt, Nx), b1, b2, seed in chains:
-        c = dsa.Chain(groups[name], D, Nt, Nx, 0.0, b1, b2, seed)
+    for name, (d, nt, nx), b1, b2, seed in chains:
+        c = dsa.Chain(groups[name], d, nt, nx, 0.0, b1, b2, seed)
         c.update(1)
         objs.append(c)
     t0 = time.perf_counter()
     for c in objs:
-        c.update(sweeps)
+        c.update(serial_sweeps)
     ts = time.perf_counter() - t0
     for c in objs:
         c.close()
+    nsw = cycles * sweeps_per_cycle
+    rb, rs = nsw * links / tb, serial_sweeps * links / ts
     return {"workload": "256 chains, 16x16 (beta1, beta2) grid, shapes 8^4/12^4/16^3x8/8^3x16 round-robin, "
-                        "192 S(1080) + 64 S(216), hot seeds 100+i", "sweeps": sweeps, "links": links,
-            "batched_link_updates_per_s": sweeps * links / tb, "serial_link_updates_per_s": sweeps * links / ts,
-            "batched_s": tb, "serial_s": ts, "speedup": ts / tb, "task_links": task_links,
+                        "192 S(1080) + 64 S(216), hot seeds 100+i",
+            "batched": f"{cycles} hipGraph replays of ({sweeps_per_cycle} sweeps + measure) = {nsw} sweeps per chain",
+            "serial": f"each chain alone, {serial_sweeps} sweeps per chain, one chain after another",
+            "links": links, "batched_link_updates_per_s": rb, "serial_link_updates_per_s": rs,
+            "batched_s": tb, "serial_s": ts, "speedup": rb / rs, "task_links": task_links,
+            "measured_plaquette_range": [float(meas[-1, :, 2].min()), float(meas[-1, :, 2].max())],
             "tuning_ms_per_sweep": {"256": tune_ms[0], "512": tune_ms[1], "1024": tune_ms[2]}}
 
 
-def serving_requests(dsa, nreq, seed=2024):
-    """LLR window requests as a Poisson stream (50 requests/s).  Each asks for
-    a_n of one plaquette-energy window of width 1 % of the S1 range (NPLAQ *
-    4.5 for S(1080), NPLAQ = 6V), centred uniformly on S1/NPLAQ in [-1.9, -0.6]
-    (the greedy drive-in from a hot start stalls in metastable states below
-    about -2.1 on 8^4); lattice 8^4, 12^4 or 16^4; beta2 in {0, -0.1, -0.25};
-    hot start; drive-in cap 4000 sweeps (llr2d.cpp's); Robbins-Monro with K =
-    20 sweeps per iteration until |delta a| < 1e-4 or 50 iterations
-    (llr.cpp:223-250)."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    arrivals = np.concatenate([[0.0], np.cumsum(rng.exponential(1 / 50.0, nreq - 1))])
-    params, shapes = [], []
-    for r in range(nreq):
-        L = int(rng.choice([8, 12, 16]))
-        b2 = float(rng.choice([0.0, -0.1, -0.25]))
-        e = float(rng.uniform(-1.9, -0.6))
-        nplaq = 6 * L ** 4
-        a0 = 1.2 + (-0.7 - e) * (1.8 / 1.6)  # rough beta at which <S1>/NPLAQ = e
-        params.append(dsa.dsa.llr_params(4, L, L, 1, b2, e * nplaq, 0.005 * 4.5 * nplaq, a0, 1000 + r, K=20, NRM=50,
-                                         tol=1e-4, cap=4000))
-        shapes.append(L)
-    return arrivals, params, shapes
-
-
-def serving(dsa, nreq, slots):
+def serving(dsa, nreq, slots, budget, serial_every):
     """p50/p99 latency (arrival to converged a_n) and windows/s of the LLR
-    serving loop, and windows/s serving the same requests one at a time."""
+    serving loop under an HBM budget, and windows/s of every
+    `serial_every`-th request served alone, one after another."""
     import numpy as np
 
     g = dsa.load_group(dsa.group_path("mys1080-v4"))
-    arrivals, params, shapes = serving_requests(dsa, nreq)
-    with dsa.dsa.LlrEngine(4, slots) as eng:
+    arrivals, params, shapes, centres = workloads.llr_requests(nreq)
+    with dsa.dsa.LlrEngine(4, slots, hbm_budget=budget) as eng:
         t0 = time.perf_counter()
         admit, done, st = eng.serve([g] * nreq, params, arrivals, chunk=5)
         makespan = time.perf_counter() - t0
     log(f"bench: served {nreq} windows in {makespan:.1f} s")
     lat = done - arrivals
+    fps = [dsa.dsa.llr_footprint(p) for p in params]
+    peak = max(sum(f for a, d, f in zip(admit, done, fps) if a <= t < d) for t in admit)
+    sample = list(range(0, nreq, serial_every))
     t_serial = 0.0
-    for r, p in enumerate(params):
+    for k, r in enumerate(sample):
         with dsa.dsa.LlrEngine(4, 1) as eng:
             t0 = time.perf_counter()
-            eng.serve([g], [p], [0.0], chunk=5)
+            eng.serve([g], [params[r]], [0.0], chunk=5)
             t_serial += time.perf_counter() - t0
-        log(f"bench: window {r + 1}/{nreq} alone, {t_serial:.1f} s so far")
+        log(f"bench: window {k + 1}/{len(sample)} alone, {t_serial:.1f} s so far")
+    served_rate = nreq / makespan
+    serial_rate = len(sample) / t_serial
     return {"workload": f"{nreq} LLR window requests, Poisson 50/s, lattices 8^4/12^4/16^4 "
                         f"({shapes.count(8)}/{shapes.count(12)}/{shapes.count(16)}), beta2 in {{0,-0.1,-0.25}}, "
-                        "window 1% of the S1 range, K=20, NRM=50, tol 1e-4",
-            "slots": slots, "p50_latency_s": float(np.percentile(lat, 50)),
-            "p99_latency_s": float(np.percentile(lat, 99)), "windows_per_s": nreq / makespan,
-            "serial_windows_per_s": nreq / t_serial, "speedup": t_serial / makespan,
-            "reached": int(sum(s["reached"] for s in st)),
+                        "window centres uniform over the S1 range [-3, 1.5] x NPLAQ, width 1% of the range, "
+                        "cold start below -1.5 per plaquette, K=20, NRM=50, tol 1e-4, drive-in cap 200",
+            "slots": slots, "hbm_budget_bytes": budget, "peak_admitted_bytes": peak,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "windows_per_s": served_rate, "serial_windows_per_s": serial_rate,
+            "serial_sample": f"every {serial_every}th request alone ({len(sample)} requests)",
+            "speedup": served_rate / serial_rate,
+            "reached": int(sum(s["reached"] for s in st)), "rejected": int(sum(s["phase"] == 4 for s in st)),
             "mean_rm_iterations": float(np.mean([s["iterations"] for s in st]))}
 
 
@@ -217,11 +283,21 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--scan-sweeps", type=int, default=10,
-                    help="sweeps of the 256-chain scan timed batched and serially (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=12,
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="OpenMP threads of the reference CPU baseline (capped by the CPUs this process may use)")
+    ap.add_argument("--beta-scan-sweeps", type=int, default=2,
+                    help="timed sweeps per point of the 32^3x64 beta grid (0: skip)")
+    ap.add_argument("--scan-cycles", type=int, default=30,
+                    help="hipGraph cycles (10 sweeps + measure) of the 256-chain scan (0: skip)")
+    ap.add_argument("--scan-serial-sweeps", type=int, default=10,
+                    help="sweeps per chain of the one-chain-at-a-time comparison")
+    ap.add_argument("--serve-requests", type=int, default=60,
                     help="LLR window requests for the serving measurement (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-budget-mb", type=int, default=64,
+                    help="HBM budget of the serving engine in MiB (admission by footprint)")
+    ap.add_argument("--serve-serial-every", type=int, default=5,
+                    help="time every n-th request alone for the one-at-a-time rate")
     ap.add_argument("--kernel", choices=["auto", "table", "class", "class-narrow", "matrix"], default="auto",
                     help="sweep kernel (identical results): element values via L2 gathers, or class values in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -296,24 +372,29 @@ def main():
         if dsa.lib().dsa_probe_copy(args.copy_probe_mb << 20, 4, C.byref(cm)) == 0:
             copy_ms = cm.value
 
+    bscan = None
+    if args.beta_scan_sweeps:
+        log(f"bench: 32^3x64 beta grid, {args.beta_scan_sweeps} sweeps per point")
+        bscan = beta_scan(dsa, g, args.beta_scan_sweeps)
+
     scan = None
-    if args.scan_sweeps:
-        log(f"bench: 256-chain scan, {args.scan_sweeps} sweeps batched and one chain at a time")
-        scan = batched_vs_serial(dsa, args.scan_sweeps)
+    if args.scan_cycles:
+        log(f"bench: 256-chain scan, {args.scan_cycles} graph cycles of 10 sweeps + measure, then one chain at a "
+            "time")
+        scan = batched_vs_serial(dsa, args.scan_cycles, 10, args.scan_serial_sweeps)
 
     serve = None
     if args.serve_requests:
-        log(f"bench: LLR serving, {args.serve_requests} requests, {args.serve_slots} slots, then one at a time")
-        serve = serving(dsa, args.serve_requests, args.serve_slots)
+        log(f"bench: LLR serving, {args.serve_requests} requests, {args.serve_slots} slots, budget "
+            f"{args.serve_budget_mb} MiB, then a sample one at a time")
+        serve = serving(dsa, args.serve_requests, args.serve_slots, args.serve_budget_mb << 20,
+                        args.serve_serial_every)
 
     cpu = None
     if not args.no_cpu_baseline:
-        try:
-            cores = min(16, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            cores = min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline, reference binary on {cores} threads")
-        cpu = cpu_baseline(gpath, cores)
+        threads = max(1, min(args.cpu_threads, host_cpu()["usable_cpus"]))
+        log(f"bench: CPU baseline, reference binary on {threads} threads, then one thread")
+        cpu = cpu_baseline(gpath, threads)
 
     out = {
         "metric": "Metropolis link-updates/s at 32^3x64 S1080; batched-vs-serial chain speedup",
@@ -338,23 +419,28 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": KERNEL_NAME[form], "avg_launch_ms": avg_launch_s * 1e3, "units_per_launch": units,
                      "algorithmic_bytes_per_unit": BYTES_PER_LINK_UPDATE,
-                     "gather": {"achieved_per_s": gather_rate, "ceiling_per_s": best,
-                                "frac": gather_rate / best if best else None,
-                                "per_unit": gathers_per_unit,
-                                # the reference's arithmetic needs 608 per link-update:
-                                # the rate it would have to sustain for this throughput
-                                "reference_equivalent_per_s": GATHERS_TABLE * value,
-                                "reference_equivalent_frac": GATHERS_TABLE * value / best if best else None},
-                     # the bound the staple-product gathers meet: L1->L2 read
-                     # requests (one 128-byte line each) per launch from the
-                     # committed counters, over the live launch time, against
-                     # one request per L2 channel per clock (16 x 8 XCDs x 2.4 GHz)
-                     "l2_requests": {"achieved_per_s": l2_req / avg_launch_s if l2_req else None,
-                                     "peak_per_s": L2_REQ_PEAK,
-                                     "frac": l2_req / avg_launch_s / L2_REQ_PEAK if l2_req else None,
-                                     "per_launch": l2_req, "l2_hit_rate": l2_hit},
+                     # what actually bounds the sweep: its table gathers, served
+                     # by L2.  L1->L2 read requests (one 128-byte line each) per
+                     # launch from the committed rocprofv3 counters, over the
+                     # live launch time, against one request per L2 channel per
+                     # clock (16 channels x 8 XCDs x 2.4 GHz); and the gather
+                     # rate against the random-gather probe's
+                     "limiter": {"kind": "l2_requests",
+                                 "frac": l2_req / avg_launch_s / L2_REQ_PEAK if l2_req else None,
+                                 "l2_requests": {"achieved_per_s": l2_req / avg_launch_s if l2_req else None,
+                                                 "peak_per_s": L2_REQ_PEAK, "per_launch": l2_req,
+                                                 "l2_hit_rate": l2_hit},
+                                 "gather": {"achieved_per_s": gather_rate, "ceiling_per_s": best,
+                                            "frac": gather_rate / best if best else None,
+                                            "per_unit": gathers_per_unit,
+                                            # the reference's arithmetic needs 608 per link-update:
+                                            # the rate it would have to sustain for this throughput
+                                            "reference_equivalent_per_s": GATHERS_TABLE * value,
+                                            "reference_equivalent_frac": GATHERS_TABLE * value / best if best
+                                            else None}},
                      "exact_redecisions_per_hit": (ex1 - ex0) / (hit1 - hit0)},
         "cpu_baseline": cpu,
+        "beta_scan": bscan,
         "batched_vs_serial": scan,
         "llr_serving": serve,
     }

@@ -213,15 +213,19 @@ void dsa::minstd_start(const Group &g, const dsa_chain_params &p, uint32_t V, st
   }
 }
 
-int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
-  std::lock_guard<std::mutex> lk(g->m);
-  auto it = g->dev.find(device);
-  if (it != g->dev.end()) {
-    out = it->second;
-    return 0;
-  }
-  DeviceTables t;
-  const dsa::Group &G = g->g;
+namespace {
+
+void free_tables(DeviceTables &t) {
+  for (void *p : {(void *)t.mult, (void *)t.inv, (void *)t.pool, (void *)t.retr, (void *)t.imtr, (void *)t.cls,
+                  (void *)t.cprod, (void *)t.rcq, (void *)t.cval, (void *)t.cval0, (void *)t.mat})
+    if (p) (void)hipFree(p);
+  t = DeviceTables();
+}
+
+// Allocates and fills every device table of `G` in `t`; on failure the
+// caller frees what was allocated.  Arrays the class kernels stage into LDS
+// in 16-byte units (cprod, cval, cval0) are padded to 16 bytes with zeros.
+int fill_tables(const dsa::Group &G, dsa_group *g, DeviceTables &t) {
   const size_t P = G.P;
   HIP_TRY(hipMalloc(&t.mult, P * P * sizeof(uint16_t)));
   HIP_TRY(hipMalloc(&t.inv, P * sizeof(uint16_t)));
@@ -244,10 +248,11 @@ int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
     HIP_TRY(hipMemcpy(t.rcq, G.rcq.data(), G.rcq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   }
   if (!G.cval.empty()) {
-    const size_t nb = dsa::pad16(G.cval.size() * sizeof(int16_t));
+    const size_t nb = dsa::pad16(G.cval.size() * sizeof(int16_t)), nb0 = dsa::pad16(P * sizeof(int16_t));
     HIP_TRY(hipMalloc(&t.cval, nb));
-    HIP_TRY(hipMalloc(&t.cval0, P * sizeof(int16_t)));
+    HIP_TRY(hipMalloc(&t.cval0, nb0));
     HIP_TRY(hipMemset(t.cval, 0, nb));
+    HIP_TRY(hipMemset(t.cval0, 0, nb0));
     HIP_TRY(hipMemcpy(t.cval, G.cval.data(), G.cval.size() * sizeof(int16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.cval0, G.cval0.data(), P * sizeof(int16_t), hipMemcpyHostToDevice));
   }
@@ -255,6 +260,23 @@ int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
   if (g->mat_state > 0) {
     HIP_TRY(hipMalloc(&t.mat, g->mat32.size() * sizeof(float)));
     HIP_TRY(hipMemcpy(t.mat, g->mat32.data(), g->mat32.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+}  // namespace
+
+int dsa::upload_tables(dsa_group *g, int device, DeviceTables &out) {
+  std::lock_guard<std::mutex> lk(g->m);
+  auto it = g->dev.find(device);
+  if (it != g->dev.end()) {
+    out = it->second;
+    return 0;
+  }
+  DeviceTables t;
+  if (int rc = fill_tables(g->g, g, t)) {
+    free_tables(t);
+    return rc;
   }
   g->dev[device] = t;
   out = t;
@@ -275,20 +297,6 @@ int cls_mode(const dsa_chain *c) {
     case DSA_KERNEL_AUTO: return c->narrow_ok && c->margin8 < 1e-3 ? (int)dsa::kClsNarrow : c->cls_best;
     default: return dsa::kClsNone;
   }
-}
-
-void free_tables(DeviceTables &t) {
-  if (t.mult) (void)hipFree(t.mult);
-  if (t.inv) (void)hipFree(t.inv);
-  if (t.pool) (void)hipFree(t.pool);
-  if (t.retr) (void)hipFree(t.retr);
-  if (t.imtr) (void)hipFree(t.imtr);
-  if (t.cls) (void)hipFree(t.cls);
-  if (t.cprod) (void)hipFree(t.cprod);
-  if (t.rcq) (void)hipFree(t.rcq);
-  if (t.cval) (void)hipFree(t.cval);
-  if (t.cval0) (void)hipFree(t.cval0);
-  t = DeviceTables();
 }
 
 int ensure_lines(dsa_chain *c, uint32_t nt, uint32_t nx) {

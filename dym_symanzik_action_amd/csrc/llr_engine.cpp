@@ -426,9 +426,12 @@ int dsa_llr_get_links(dsa_llr *L, uint32_t slot, int32_t *a) {
 
 // Serving: requests arrive at arrival_s[r] seconds after the call starts; each
 // is admitted into a free slot once it has arrived and its footprint fits the
-// budget, advanced `chunk` sweeps per round together with every other active
-// window, and retired as soon as its Robbins-Monro iteration converges (or its
-// drive-in fails).  done_s[r] is the retirement time, status[r] its final state.
+// budget (in arrival order: a request that must wait holds back the ones
+// behind it), advanced `chunk` sweeps per round together with every other
+// active window, and retired as soon as its Robbins-Monro iteration converges
+// (or its drive-in fails).  done_s[r] is the retirement time, status[r] its
+// final state; a request whose footprint exceeds the whole budget, or whose
+// parameters are refused, is retired on arrival with phase DSA_LLR_REJECTED.
 int dsa_llr_serve(dsa_llr *L, const dsa_group *const *groups, const dsa_llr_params *req, const double *arrival_s,
                   uint32_t nreq, uint32_t chunk, double *admit_s, double *done_s, dsa_llr_status *status) {
   if (!L || !groups || !req || !arrival_s || !done_s || !status || !nreq)
@@ -443,14 +446,34 @@ int dsa_llr_serve(dsa_llr *L, const dsa_group *const *groups, const dsa_llr_para
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return arrival_s[x] < arrival_s[y]; });
   size_t next = 0;
   uint32_t finished = 0, active = 0;
+  // a request that no retirement can make room for (its footprint above the
+  // budget, or parameters the engine refuses) retires at once as REJECTED
+  auto reject = [&](uint32_t r) {
+    memset(&status[r], 0, sizeof status[r]);
+    status[r].phase = DSA_LLR_REJECTED;
+    const double t = now();
+    if (admit_s) admit_s[r] = t;
+    done_s[r] = t;
+    ++finished;
+  };
   while (finished < nreq) {
-    // admit everything that has arrived and fits
+    // admit, in arrival order, everything that has arrived and fits
     while (next < nreq && arrival_s[order[next]] <= now()) {
       const uint32_t r = order[next];
+      if (!groups[r] || (L->budget && footprint(req[r]) > L->budget)) {
+        reject(r);
+        ++next;
+        continue;
+      }
       uint32_t s;
       int rc = dsa_llr_add(L, groups[r], &req[r], nullptr, &s);
-      if (rc == DSA_E_RANGE && active) break;  // full: wait for a retirement
-      if (rc) return rc;                       // cannot fit even alone
+      if (rc == DSA_E_RANGE && active) break;  // slots or budget full: wait for a retirement
+      if (rc == DSA_E_ARG || rc == DSA_E_RANGE || rc == DSA_E_STATE) {
+        reject(r);
+        ++next;
+        continue;
+      }
+      if (rc) return rc;  // device failure
       slot_of[r] = (int)s;
       req_of[s] = (int)r;
       if (admit_s) admit_s[r] = now();

@@ -486,7 +486,17 @@ class LlrStatus(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-LLR_PHASE = {0: "seed", 1: "rm", 2: "done", 3: "idle"}
+LLR_PHASE = {0: "seed", 1: "rm", 2: "done", 3: "idle", 4: "rejected"}
+
+
+def llr_footprint(params: "LlrParams") -> int:
+    """Device bytes one LLR chain of these parameters holds (what serving
+    admission counts against the engine's budget)."""
+    L = lib()
+    L.dsa_llr_footprint.argtypes = [C.POINTER(LlrParams), C.POINTER(C.c_uint64)]
+    n = C.c_uint64()
+    _check(L.dsa_llr_footprint(C.byref(params), C.byref(n)))
+    return n.value
 
 
 def llr_params(D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, start="hot", K=20, NRM=50, tol=0.0, cap=0, Ewin2=0.0,

@@ -238,7 +238,9 @@ typedef struct {
   double Ewin2, hw2, a20;
 } dsa_llr_params;
 
-enum { DSA_LLR_SEED = 0, DSA_LLR_RM = 1, DSA_LLR_DONE = 2, DSA_LLR_IDLE = 3 };
+/* DSA_LLR_REJECTED: a served request refused at admission (footprint above
+ * the engine's whole budget, or parameters the engine refuses). */
+enum { DSA_LLR_SEED = 0, DSA_LLR_RM = 1, DSA_LLR_DONE = 2, DSA_LLR_IDLE = 3, DSA_LLR_REJECTED = 4 };
 
 typedef struct {
   double a;           /* current tilt a_n */
@@ -275,10 +277,14 @@ int dsa_llr_step(dsa_llr *L, uint32_t nsweeps);
 int dsa_llr_status_get(dsa_llr *L, uint32_t slot, dsa_llr_status *out);
 int dsa_llr_get_links(dsa_llr *L, uint32_t slot, int32_t *a);
 /* Continuous-batching service of nreq window requests arriving at arrival_s[r]
- * seconds: admission by free slot and footprint, `chunk` sweeps per round for
- * every active chain, retirement on convergence.  Outputs per request: admit
- * and done times (seconds from the call) and the final status.  (The
- * reference serves windows one process at a time: scripts/llr2d_parallel.sh.) */
+ * seconds: admission in arrival order by free slot and footprint
+ * (dsa_llr_footprint; the sum over admitted chains never exceeds the
+ * engine's hbm_budget), `chunk` sweeps per round for every active chain,
+ * retirement on convergence.  Outputs per request: admit and done times
+ * (seconds from the call) and the final status; a request that can never
+ * fit is retired on arrival with phase DSA_LLR_REJECTED and the others are
+ * served.  Returns an error only for a device failure.  (The reference
+ * serves windows one process at a time: scripts/llr2d_parallel.sh:28-37.) */
 int dsa_llr_serve(dsa_llr *L, const dsa_group *const *groups, const dsa_llr_params *req, const double *arrival_s,
                   uint32_t nreq, uint32_t chunk, double *admit_s, double *done_s, dsa_llr_status *status);
 
