@@ -659,6 +659,43 @@ int dsa_chain_measure(dsa_chain *c, uint32_t wk, uint32_t wl, int32_t sum_mode, 
   return 0;
 }
 
+int dsa_chain_measure_noalloc(dsa_chain *c, uint32_t wk, uint32_t wl, int32_t sum_mode, double *out) {
+  static_assert(dsa::kScratchBytes == DSA_NOALLOC_SCRATCH_BYTES, "work space size differs from the header's");
+  if (!c || !out) return fail(DSA_E_ARG, "dsa_chain_measure_noalloc: NULL argument");
+  if (wk > c->p.Nx || wl > c->p.Nt) return fail(DSA_E_ARG, "Wilson grid larger than Nx x Nt");
+  if ((wk == 0) != (wl == 0)) return fail(DSA_E_ARG, "Wilson grid must be empty or at least 1x1");
+  if (sum_mode != DSA_SUM_TREE && sum_mode != DSA_SUM_ORDERED) return fail(DSA_E_ARG, "unknown sum mode");
+  // the work space: result staging, per-block partial sums (tree), line
+  // products (Wilson grid), each region 256-byte aligned
+  auto pad256 = [](size_t n) { return (n + 255) & ~(size_t)255; };
+  const uint32_t nobs = 4 + wk * wl;
+  const size_t out_b = pad256((size_t)nobs * sizeof(double));
+  const size_t part_b = sum_mode == DSA_SUM_TREE ? pad256((size_t)nobs * dsa::obs_blocks(c->G) * sizeof(double)) : 0;
+  const size_t lines_b = (wk && wl) ? ((size_t)wl + (size_t)(c->G.D - 1) * wk) * c->G.V * sizeof(uint16_t) : 0;
+  const size_t need = out_b + part_b + lines_b;
+  if (need > dsa::kScratchBytes)
+    return fail(DSA_E_RANGE, "dsa_chain_measure_noalloc: this lattice and Wilson grid need " + std::to_string(need) +
+                                 " bytes of work space, above the " + std::to_string(dsa::kScratchBytes) +
+                                 "-byte limit (use dsa_chain_measure)");
+  if (set_device(c)) return DSA_E_DEVICE;
+  void *base = nullptr;
+  HIP_TRY(dsa::scratch_region(&base));
+  unsigned char *ws = static_cast<unsigned char *>(base);
+  double *d_out = reinterpret_cast<double *>(ws);
+  double *partials = reinterpret_cast<double *>(ws + out_b);
+  uint16_t *lines = (wk && wl) ? reinterpret_cast<uint16_t *>(ws + out_b + part_b) : nullptr;
+  // every region is written before it is read: the line kernel fills all
+  // rows x V products, every observable block writes its partial sums
+  if (lines) HIP_TRY(dsa::launch_lines(lines, c->d_links, c->T, c->G, wl, wk, c->stream));
+  if (sum_mode == DSA_SUM_TREE)
+    HIP_TRY(dsa::launch_measure_tree(d_out, partials, c->d_links, lines, c->T, c->G, wk, wl, wl, wk, c->stream));
+  else
+    HIP_TRY(dsa::launch_measure_ordered(d_out, c->d_links, lines, c->T, c->G, wk, wl, wl, wk, c->stream));
+  HIP_TRY(hipMemcpyAsync(out, d_out, nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int dsa_chain_get_links(dsa_chain *c, int32_t *a) {
   if (!c || !a) return fail(DSA_E_ARG, "dsa_chain_get_links: NULL argument");
   if (set_device(c)) return DSA_E_DEVICE;

@@ -7,6 +7,8 @@
 // the proposal pool in LDS.  No MFMA: there is no dense contraction here.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -719,6 +721,31 @@ hipError_t launch_measure_ordered(double *out, const uint16_t *links, const uint
                                   hipStream_t s) {
   hipLaunchKernelGGL(k_obs_ordered, dim3(1), dim3(64), 0, s, out, links, lines, T, G, wk, wl, nmax_t, nmax_x);
   return hipGetLastError();
+}
+
+// The device work space of the entry points that allocate nothing
+// (dsa_chain_measure_noalloc): one array per GPU, loaded with the code object
+// and held until the process ends.  Host code never passes `scratch` itself
+// to a kernel (on the host it names a stand-in, not device memory): it passes
+// the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+hipError_t scratch_region(void **ptr) {
+  static std::mutex m;
+  static std::map<int, void *> addr;  // device -> address of its copy of `scratch`
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(m);
+  auto it = addr.find(dev);
+  if (it == addr.end()) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    it = addr.emplace(dev, p).first;
+  }
+  *ptr = it->second;
+  return hipSuccess;
 }
 
 }  // namespace dsa

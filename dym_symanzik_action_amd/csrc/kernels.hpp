@@ -224,6 +224,10 @@ uint32_t obs_blocks(const Geom &G);
 hipError_t launch_measure_tree(double *out, double *partials, const uint16_t *links, const uint16_t *lines,
                                const Tables &T, const Geom &G, uint32_t wk, uint32_t wl, uint32_t nmax_t,
                                uint32_t nmax_x, hipStream_t s);
+// Bytes of the per-GPU static work space of the entry points that allocate
+// nothing, and its address on the current device (looked up once per device).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+hipError_t scratch_region(void **ptr);
 // The reference's one-thread order (dym-mod-metro.cpp:282-318): one lane.
 hipError_t launch_measure_ordered(double *out, const uint16_t *links, const uint16_t *lines, const Tables &T,
                                   const Geom &G, uint32_t wk, uint32_t wl, uint32_t nmax_t, uint32_t nmax_x,

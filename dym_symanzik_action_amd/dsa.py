@@ -73,6 +73,7 @@ _SIGS = {
     "dsa_chain_get_sweep": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "dsa_chain_set_sweep": (C.c_int, [C.c_void_p, C.c_uint64]),
     "dsa_chain_measure": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p]),
+    "dsa_chain_measure_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p]),
     "dsa_chain_get_links": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dsa_chain_set_links": (C.c_int, [C.c_void_p, C.c_void_p]),
     "dsa_chain_get_rng_state": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -295,6 +296,16 @@ class Chain:
         wl = self.Nt if wl is None else wl
         out = np.empty(4 + wk * wl, np.float64)
         _check(lib().dsa_chain_measure(self._h, wk, wl, DSA_SUM[sum], _ptr(out)))
+        return out
+
+    def measure_noalloc(self, wk: Optional[int] = None, wl: Optional[int] = None, sum: str = "tree") -> np.ndarray:
+        """measure() with no device allocation: the work space is the
+        library's static per-GPU array (raises DSAError DSA_E_RANGE when the
+        lattice and grid need more than it holds)."""
+        wk = self.Nx if wk is None else wk
+        wl = self.Nt if wl is None else wl
+        out = np.empty(4 + wk * wl, np.float64)
+        _check(lib().dsa_chain_measure_noalloc(self._h, wk, wl, DSA_SUM[sum], _ptr(out)))
         return out
 
     # -- the gauge field a[i*D+mu] (lattice.h:9)

@@ -90,7 +90,23 @@ int dsa_chain_set_sweep(dsa_chain *c, uint64_t sweep);
  * DSA_SUM_TREE: fixed-shape FP64 tree (deterministic); DSA_SUM_ORDERED: the
  * reference's one-thread summation order (one lane; small lattices). */
 int dsa_chain_measure(dsa_chain *c, uint32_t wk, uint32_t wl, int32_t sum_mode, double *out);
-/* Read / write the gauge field `a` (lattice.h:9, lattice.c:7,15). */
+/* dsa_chain_measure without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code between HIP graph
+ * captures, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as dsa_chain_measure (bit-identical), but the
+ * per-block partial sums, the Wilson-loop line products and the result
+ * staging live in a library-owned static device array of
+ * DSA_NOALLOC_SCRATCH_BYTES per GPU instead of memory the chain allocates
+ * on first use; calls no hipMalloc/hipFree.  DSA_E_RANGE when the lattice
+ * and grid need more: 8*(4+wk*wl)*(1 + blocks) bytes (blocks <= 1024, tree
+ * mode) plus 2*V*(wl + (D-1)*wk) for a Wilson grid, each region rounded up
+ * to 256.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (chains measured concurrently from several threads) race and
+ * must be serialised by the caller; each call has finished with the array
+ * when it returns. */
+#define DSA_NOALLOC_SCRATCH_BYTES (32u * 1024u * 1024u)
+int dsa_chain_measure_noalloc(dsa_chain *c, uint32_t wk, uint32_t wl, int32_t sum_mode, double *out);
+/* Read / write the gauge field `a` (lattice.h:9, lattice.c:7,15). *//* Read / write the gauge field `a` (lattice.h:9, lattice.c:7,15). */
 int dsa_chain_get_links(dsa_chain *c, int32_t *a);
 int dsa_chain_set_links(dsa_chain *c, const int32_t *a);
 /* Minstd per-site stream states (V values) for checkpointing in that mode. */
