@@ -149,3 +149,50 @@ void ogp_llr_links(void *Lv, int32_t *out) {
   ogl_dev *L = (ogl_dev *)Lv;
   memcpy(out, L->c.L.a, (size_t)L->c.L.V * L->c.L.D * 4);
 }
+
+/* ---- the reference llr's serial chain (ogl_ref_*), step by step ---- */
+void *ogp_lref_new(void *g, unsigned D, unsigned Nt, unsigned Nx, int axis, double bg, int seed) {
+  ogl_ref *R = (ogl_ref *)calloc(1, sizeof *R);
+  if (ogl_ref_init(R, (og_group *)g, D, Nt, Nx, axis, bg, seed)) {
+    free(R);
+    return NULL;
+  }
+  return R;
+}
+void ogp_lref_free(void *Rv) {
+  if (Rv) {
+    og_chain_free(&((ogl_ref *)Rv)->c);
+    free(Rv);
+  }
+}
+double ogp_lref_rect_mult(void *Rv) { return ((ogl_ref *)Rv)->rect_mult; }
+/* llr.cpp:326: rnd[i].seed(iseed + r*1000003 + i) */
+void ogp_lref_reseed(void *Rv, unsigned r) {
+  ogl_ref *R = (ogl_ref *)Rv;
+  for (unsigned i = 0; i < R->c.L.V; ++i)
+    R->c.ms[i] = og_minstd_seed((long long)(uint32_t)((uint32_t)R->c.seed + r * 1000003u + i));
+}
+/* run_pass's start (llr.cpp:258-260): hot links from site 0's stream or cold */
+void ogp_lref_start(void *Rv, int hot) {
+  ogl_ref *R = (ogl_ref *)Rv;
+  og_lat *L = &R->c.L;
+  const size_t nl = (size_t)L->V * L->D;
+  for (size_t i = 0; i < nl; ++i) L->a[i] = hot ? og_randint(&R->c.ms[0], 0, (int)L->g->P - 1) : L->g->id;
+  R->S1 = og_S1(L);
+  R->S2 = og_sum_m1(L) / R->rect_mult;
+}
+int ogp_lref_seed_window(void *Rv, double Elo, double Ehi, double a_guess) {
+  return ogl_ref_seed_window((ogl_ref *)Rv, Elo, Ehi, a_guess);
+}
+double ogp_lref_rm_solve(void *Rv, double Ewin, double hw, double a0, unsigned K, unsigned NRM) {
+  return ogl_ref_rm_solve((ogl_ref *)Rv, Ewin, hw, a0, K, NRM);
+}
+/* out: S1, S2, maxdrift, hit, acc */
+void ogp_lref_state(void *Rv, double *out) {
+  ogl_ref *R = (ogl_ref *)Rv;
+  out[0] = R->S1; out[1] = R->S2; out[2] = R->maxdrift; out[3] = (double)R->c.hit; out[4] = (double)R->c.acc;
+}
+void ogp_lref_links(void *Rv, int32_t *out) {
+  ogl_ref *R = (ogl_ref *)Rv;
+  memcpy(out, R->c.L.a, (size_t)R->c.L.V * R->c.L.D * 4);
+}

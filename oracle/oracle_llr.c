@@ -154,8 +154,13 @@ static int run_pass(ogl_ref *R, int dir, double a0, double Etop, double stepE, d
 int ogl_ref_init(ogl_ref *R, const og_group *g, unsigned D, unsigned Nt, unsigned Nx, int axis, double bg,
                  int seed) {
   memset(R, 0, sizeof *R);
-  /* the chain's init draws the hot links from site 0 (llr.cpp:314-315) */
-  if (og_chain_init(&R->c, g, D, Nt, Nx, 0, 0, 0, seed, 0, OG_RNG_MINSTD, OG_ORDER_REFERENCE)) return -1;
+  if (og_chain_init(&R->c, g, D, Nt, Nx, 0, 0, 0, seed, 1, OG_RNG_MINSTD, OG_ORDER_REFERENCE)) return -1;
+  /* llr.cpp:314-315: rnd[i].seed(iseed + i) with i unsigned, so the seed is
+   * the 32-bit unsigned sum, zero-extended (dym-mod-metro's int i
+   * sign-extends); then the hot links from site 0's stream */
+  const unsigned V = R->c.L.V;
+  for (unsigned i = 0; i < V; ++i) R->c.ms[i] = og_minstd_seed((long long)(uint32_t)((uint32_t)seed + i));
+  for (size_t l = 0; l < (size_t)V * D; ++l) R->c.L.a[l] = og_randint(&R->c.ms[0], 0, (int)g->P - 1);
   R->axis = axis;
   R->bg = bg;
   R->rect_mult = 6.0;
@@ -199,7 +204,7 @@ int ogl_ref_main(int argc, char **argv) {
   double *an = (double *)calloc((size_t)M * 2 * Rr, sizeof(double));
   int *nan_ = (int *)calloc((size_t)M, sizeof(int));
   for (unsigned r = 0; r < Rr; ++r) {
-    for (unsigned i = 0; i < V; ++i) R.c.ms[i] = og_minstd_seed((long long)(int)(iseed + (int)(r * 1000003u) + (int)i));
+    for (unsigned i = 0; i < V; ++i) R.c.ms[i] = og_minstd_seed((long long)(uint32_t)((uint32_t)iseed + r * 1000003u + i));
     int rd = run_pass(&R, +1, a0, Etop, stepE, hw, K, NRM, M, an, nan_, 2 * Rr);
     int ru = run_pass(&R, -1, a0, Etop, stepE, hw, K, NRM, M, an, nan_, 2 * Rr);
     printf("repeat %u: down reached %d/%d, up reached %d/%d\n", r, rd, M, ru, M);

@@ -252,3 +252,65 @@ class OLlr:
         if getattr(self, "h", None):
             lib().ogp_llr_free(self.h)
             self.h = None
+
+
+class OLlrRef:
+    """The reference llr's serial chain restated (oracle_llr.c, ogl_ref_*):
+    main's set-up with RECT_MULT calibrated (llr.cpp:296-321), then run_pass's
+    steps (llr.cpp:223-274) one call at a time."""
+
+    def __init__(self, g: OGroup, D, Nt, Nx, axis, bg, seed):
+        L = lib()
+        if not hasattr(L.ogp_lref_new, "_sig"):
+            vp, u32, i32, dbl = C.c_void_p, C.c_uint32, C.c_int32, C.c_double
+            L.ogp_lref_new.restype = vp
+            L.ogp_lref_new.argtypes = [vp, u32, u32, u32, i32, dbl, i32]
+            L.ogp_lref_free.argtypes = [vp]
+            L.ogp_lref_rect_mult.restype = dbl
+            L.ogp_lref_rect_mult.argtypes = [vp]
+            L.ogp_lref_reseed.argtypes = [vp, u32]
+            L.ogp_lref_start.argtypes = [vp, i32]
+            L.ogp_lref_seed_window.restype = C.c_int
+            L.ogp_lref_seed_window.argtypes = [vp, dbl, dbl, dbl]
+            L.ogp_lref_rm_solve.restype = dbl
+            L.ogp_lref_rm_solve.argtypes = [vp, dbl, dbl, dbl, u32, u32]
+            L.ogp_lref_state.argtypes = [vp, vp]
+            L.ogp_lref_links.argtypes = [vp, vp]
+            L.ogp_lref_new._sig = True
+        self.g = g
+        self.h = L.ogp_lref_new(g.h, D, Nt, Nx, axis, bg, seed)
+        if not self.h:
+            raise ValueError("oracle: llr reference chain init failed")
+        self.V, self.D = Nt * Nx ** (D - 1), D
+
+    @property
+    def rect_mult(self):
+        return lib().ogp_lref_rect_mult(self.h)
+
+    def reseed(self, r):
+        lib().ogp_lref_reseed(self.h, r)
+
+    def start(self, hot):
+        lib().ogp_lref_start(self.h, 1 if hot else 0)
+
+    def seed_window(self, Elo, Ehi, a_guess):
+        return bool(lib().ogp_lref_seed_window(self.h, Elo, Ehi, a_guess))
+
+    def rm_solve(self, Ewin, hw, a0, K, NRM):
+        return lib().ogp_lref_rm_solve(self.h, Ewin, hw, a0, K, NRM)
+
+    def state(self):
+        out = np.empty(5, np.float64)
+        lib().ogp_lref_state(self.h, _p(out))
+        return {"S1": out[0], "S2": out[1], "maxdrift": out[2], "hit": int(out[3]), "acc": int(out[4])}
+
+    @property
+    def a(self):
+        out = np.empty(self.V * self.D, np.int32)
+        lib().ogp_lref_links(self.h, _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ogp_lref_free(self.h)
+            self.h = None

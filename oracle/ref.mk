@@ -10,6 +10,8 @@
 #   _ref/dym-mod-metro-savecfg  the driver that also writes SU(3) configurations
 #   _ref/ref_harness     ref_harness.cpp (ours) linked against the reference's
 #                        own get_staples / get_rect_loops / update (main renamed)
+#   _ref/ref_llr_harness ref_llr_harness.cpp (ours) linked against llr.cpp's own
+#                        calibrate_rect_mult / seed_window / rm_solve (main renamed)
 #   _ref/verify_group    the reference's table checker
 REF ?= /root/reference
 HERE := $(patsubst %/,%,$(dir $(lastword $(MAKEFILE_LIST))))
@@ -23,7 +25,7 @@ LDLIBS = -lm
 SRC_C = $(REF)/lattice.c $(REF)/group.c
 
 all: $(OUT)/dym-mod-metro $(OUT)/llr $(OUT)/llr2d $(OUT)/ref_harness $(OUT)/verify_group \
-     $(OUT)/dym-mod-metro-savecfg
+     $(OUT)/dym-mod-metro-savecfg $(OUT)/ref_llr_harness
 
 $(OUT):
 	mkdir -p $(OUT)
@@ -53,6 +55,11 @@ $(OUT)/metro_nomain.o: $(REF)/dym-mod-metro.cpp | $(OUT)
 	g++ -O3 -fopenmp -std=c++11 -w -Dmain=ref_main -I$(REF) -c $< -o $@
 $(OUT)/ref_harness: $(HERE)/ref_harness.cpp $(OUT)/metro_nomain.o $(REF_OBJS)
 	g++ -O2 -fopenmp -std=c++11 -I$(REF) $< $(OUT)/metro_nomain.o $(REF_OBJS) $(LDLIBS) -pg -o $@
+
+$(OUT)/llr_nomain.o: $(REF)/llr.cpp | $(OUT)
+	g++ -O3 -fopenmp -std=c++11 -w -Dmain=llr_main -I$(REF) -c $< -o $@
+$(OUT)/ref_llr_harness: $(HERE)/ref_llr_harness.cpp $(OUT)/llr_nomain.o $(REF_OBJS)
+	g++ -O2 -fopenmp -std=c++11 -I$(REF) $< $(OUT)/llr_nomain.o $(REF_OBJS) $(LDLIBS) -pg -o $@
 
 clean:
 	rm -rf $(OUT)

@@ -157,9 +157,28 @@ def llr2d_fixture(name, group, D, Nt, Nx, seed_args, solve_args):
                         stats_a1=np.array(a1s), stats_a2=np.array(a2s), stats_seeds=np.arange(100, 108), **cfgs)
 
 
+def llr_ladder_fixture(name, args):
+    """The reference llr's ladder window by window (oracle/_ref/ref_llr_harness
+    ladder: main's set-up, then one repeat of run_pass down and up calling
+    llr.cpp's own seed_window / rm_solve): RECT_MULT, every window's energies
+    and a_n as exact doubles, the links at each pass's start and after each
+    window.  args = group D Nt Nx axis bg Etop Ebot step hw a0 seed K NRM."""
+    with tempfile.TemporaryDirectory() as td:
+        run([os.path.join(REFBIN, "ref_llr_harness"), "ladder", os.path.join(REF, "groups", args[0])] + args[1:] +
+            [td])
+        ld = lambda f, t: np.fromfile(os.path.join(td, f), dtype=t)
+        win = ld("windows.f64", np.float64).reshape(-1, 9)
+        links = {}
+        for f in sorted(os.listdir(td)):
+            if f.endswith(".i32"):
+                links[f[:-4]] = ld(f, np.int32)
+        np.savez_compressed(os.path.join(HERE, f"llrladder_{name}.npz"), args=np.array(args),
+                            rect_mult=ld("init.f64", np.float64)[0], windows=win, **links)
+
+
 def main():
     """Writes every fixture, or only the groups named on the command line
-    (loops, traj, cli, replicas, llr, files)."""
+    (loops, traj, cli, replicas, llr, llrladder, files)."""
     if not os.path.exists(os.path.join(REFBIN, "ref_harness")):
         sys.exit("build the reference first: make -f oracle/ref.mk")
     only = set(sys.argv[1:])
@@ -197,6 +216,22 @@ def main():
                                    "3", "1"])
         llr_fixture("stats_4x4", [g, "4", "4", "4", "1", "-0.1", "-1500", "-1500", "100", "75", "1.5", "20", "20",
                                   "30", "8"])
+    if want("llrladder"):  # llr's ladders window by window, exact (the replay's pins)
+        llr_ladder_fixture("axis1_3x3", [g, "4", "3", "3", "1", "-0.1", "-500", "-800", "150", "60", "-1.0", "5", "3",
+                                         "2"])
+        llr_ladder_fixture("axis2_4x3", [g, "4", "4", "3", "2", "1.2", "-300", "-500", "100", "50", "0.5", "6", "2",
+                                         "2"])
+        llr_ladder_fixture("s216_axis1_3x4", ["S216", "4", "3", "4", "1", "0.0", "-900", "-1000", "100", "60", "1.0",
+                                              "8", "2", "3"])
+        # a negative seed: llr seeds its engines with the unsigned sum iseed + i
+        llr_ladder_fixture("axis1_3x3_negseed", [g, "4", "3", "3", "1", "0.0", "-450", "-600", "150", "60", "-0.8",
+                                                 "-7", "2", "2"])
+        # extent-2 lattices: rectangles wrap, RECT_MULT != 6 (7.77 on 2^4 and 7.69
+        # on 2^3 x 3); the second ladder's up pass stops at an unreached window
+        llr_ladder_fixture("axis1_2x2", [g, "4", "2", "2", "1", "0.0", "-100", "-150", "50", "20", "0.5", "9", "2",
+                                         "2"])
+        llr_ladder_fixture("axis2_3x2", [g, "4", "3", "2", "2", "0.8", "-50", "-100", "50", "25", "0.5", "12", "2",
+                                         "2"])
     if want("files"):  # configuration files: savecfg's SU(3) output on config 1, llr2d's cells
         kentucky_fixture("c1", g, ["4", "2", "2", "0", "1.5", "0.1"], 1, 3, 2)
         llr2d_fixture("seed_3x3", g, 4, 3, 3,

@@ -7,9 +7,10 @@
   binary's GMES output (cli_c1.txt);
 * Philox streams in the race-free colour order: bit-exact against the oracle;
 * FP64 observable sums: <= 1e-12 * sum|terms| against the reference's order;
-* statistics of the default mode: within 3.5 sigma (Gamma-method errors) of
-  the reference binary's runs (cli_stats_*.txt); the four stream x order
-  combinations sample one ensemble;
+* statistics of the default mode: the reference binary's runs
+  (cli_stats_*.txt) agree with it at the family-wise level of one two-sided
+  2-sigma test over all observables compared together (Bonferroni, Gamma-
+  method errors); the four stream x order combinations sample one ensemble;
 * at the 32^3 x 64 benchmark size: size-independent properties; at pass
   sizes between the small-pass and device-filling class launches, the class
   kernel equals the table kernel bit for bit.
@@ -303,11 +304,11 @@ def _stats_compare(ref_lines, got_lines, nx, nt, keys):
 ])
 def test_statistics_agree_with_reference(fixture, args, nx, nt):
     """Long single chains in a mixing regime (acceptance 0.2-0.3): means of the
-    plaquette, rectangle, |P| and Wilson loops agree within 3.5 sigma, sigma
-    from both chains' Gamma-method (autocorrelation-corrected) errors.  Eight
-    correlated observables are tested at once, so a 2-sigma cut would fail a
-    correct sampler about one run in three; a wrong action or staple shifts
-    these means by tens of sigma."""
+    plaquette, rectangle, |P| and Wilson loops agree, sigma from both chains'
+    Gamma-method (autocorrelation-corrected) errors.  Eight observables are
+    tested together at the family-wise level of one two-sided 2-sigma test
+    (Bonferroni: |z| < 2.77 each); a wrong action or staple shifts these
+    means by tens of sigma."""
     say("case stats", fixture)
     out = subprocess.run([CLI, group_file("mys1080-v4")] + args, capture_output=True, text=True, check=True,
                          timeout=900).stdout
@@ -316,7 +317,9 @@ def test_statistics_agree_with_reference(fixture, args, nx, nt):
     rep = _stats_compare(ref, out.splitlines(), nx, nt, keys)
     for k, m1, e1, m2, e2, z in rep:
         say(f"  {k}: ref {m1:.6f}+-{e1:.6f}  gpu {m2:.6f}+-{e2:.6f}  z={z:.2f}")
-    assert all(z < 3.5 for *_, z in rep), rep
+    from stats_util import z_limit
+
+    assert all(z < z_limit(len(keys)) for *_, z in rep), rep
 
 
 def test_visiting_orders_and_streams_sample_the_same_ensemble(dsa, dgroups):
@@ -324,11 +327,13 @@ def test_visiting_orders_and_streams_sample_the_same_ensemble(dsa, dgroups):
     free colours, the reference's one-thread order) are different Markov chains
     with one stationary distribution: 4 x 32 independent 4^4 chains (beta =
     (0, 1.2, -0.1), acceptance ~0.29), 100 + 2 x 2000 sweeps each, agree on
-    the plaquette and rectangle means to ~4e-5 (3.5 sigma of the difference).
+    the plaquette and rectangle means: 12 comparisons (2 observables x 6
+    pairs) at the family-wise level of one two-sided 2-sigma test
+    (Bonferroni: |z| < 2.90 each).
     minstd-compat seeds site i's engine with iseed + i (dym-mod-metro.cpp:
     227-228), so chains with nearby seeds run the same engines one site apart
     and are not independent: their seeds are spaced 1000003 apart."""
-    from stats_util import gamma_method
+    from stats_util import gamma_method, z_limit
 
     say("case orders x streams ensemble")
     g = dgroups("mys1080-v4")
@@ -355,7 +360,9 @@ def test_visiting_orders_and_streams_sample_the_same_ensemble(dsa, dgroups):
         for a in range(4):
             for c in range(a + 1, 4):
                 (m1, e1), (m2, e2) = res[a][q], res[c][q]
-                assert abs(m1 - m2) < 3.5 * np.hypot(e1, e2), (modes[a], modes[c], q, m1, m2, e1, e2)
+                z = abs(m1 - m2) / np.hypot(e1, e2)
+                say(f"  {modes[a]} vs {modes[c]} q={q}: z={z:.2f}")
+                assert z < z_limit(12), (modes[a], modes[c], q, m1, m2, e1, e2)
 
 
 def test_frozen_config1_replicas_agree_with_reference(dsa, dgroups):

@@ -113,7 +113,7 @@ def test_llr_ladders_match_oracle_replay(oracle, ogroups):
 def test_llr_statistics_match_reference_llr_cli():
     """tests/golden/llr_stats_4x4.txt's command line (4^4, one window, R = 8
     repeats of both ladders): the drop-in's a_n agrees with the reference's
-    within 3.5 standard errors."""
+    (Welch's t-test over the 16 ladders each, two-sided 2-sigma level)."""
     ref = open(os.path.join(GOLDEN, "llr_stats_4x4.txt")).read().splitlines()
     args = ref[0].split()[2:-1]
     say("case llr cli statistics", args)
@@ -122,9 +122,11 @@ def test_llr_statistics_match_reference_llr_cli():
     theirs = [l for l in ref if l.startswith("ANE:")][0].split()
     assert mine[1] == theirs[1] and mine[4] == theirs[4] == "16"
     m1, s1, m2, s2 = float(mine[2]), float(mine[3]), float(theirs[2]), float(theirs[3])
-    z = abs(m1 - m2) / np.hypot(s1 / 4, s2 / 4)
-    say(f"  a_n device {m1} +- {s1}, reference {m2} +- {s2}: z = {z:.2f}")
-    assert z < 3.5
+    from stats_util import p_limit, welch_p
+
+    p = welch_p(m1, s1, 16, m2, s2, 16)
+    say(f"  a_n device {m1} +- {s1}, reference {m2} +- {s2}: p = {p:.3f}")
+    assert p > p_limit(1)
     assert sum(l.startswith("repeat ") for l in out.splitlines()) == 8
 
 
