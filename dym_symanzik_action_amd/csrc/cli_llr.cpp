@@ -5,6 +5,9 @@
 //         (llr.cpp:276-345: R repeats of a hot ladder walking down and a cold
 //         ladder walking up the windows, each window seeded from the previous
 //         window's configuration and tilt; ANE lines)
+//   llr --replay group D Nt Nx ... (the same arguments): the reference's own
+//         serial chain replayed on the device (dsa_llr_ref_*), one window
+//         after another: every line equals the reference's at one thread
 //   llr2d group D Nt Nx E1top E1bot step1 hw1 c0 c1 c2 step2 hw2 nperp a0 seed [K] [NRM] [R]
 //   llr2d seed  group D Nt Nx E1top E1bot step1 hw1 c0 c1 c2 step2 hw2 nperp a0 seed cfgdir
 //   llr2d solve group D Nt Nx manifest lo hi a0 K NRM seed
@@ -81,6 +84,8 @@ class Runner {
   bool ok() const { return ok_; }
   void submit(Task t) { queue_.push_back(std::move(t)); }
   double maxdrift() const { return maxdrift_; }
+  double drift1() const { return drift1_; }
+  double drift2() const { return drift2_; }
 
   int run() {
     std::vector<Task *> slot_task(nslots_, nullptr);
@@ -109,6 +114,8 @@ class Runner {
         r.a2 = st.a2;
         r.maxdrift = st.maxdrift;
         maxdrift_ = std::max(maxdrift_, st.maxdrift);
+        drift1_ = std::max(drift1_, st.drift1);
+        drift2_ = std::max(drift2_, st.drift2);
         const dsa_llr_params &p = slot_task[s]->p;
         uint64_t V = p.Nt;
         for (uint32_t d = 1; d < p.D; ++d) V *= p.Nx;
@@ -136,16 +143,37 @@ class Runner {
   bool ok_ = false;
   bool trace_ = false;
   std::vector<Task> queue_;
-  double maxdrift_ = 0;
+  double maxdrift_ = 0, drift1_ = 0, drift2_ = 0;
 };
 
-// The reference calibrates the rectangle multiplicity from one link flip
-// (llr.cpp:152-171); the engine counts every 1x2 rectangle once, which is the
-// calibrated value 6 wherever no rectangle wraps onto itself (extents >= 3).
-// The lines the reference's lattice set-up prints (lattice.c:57).
-void print_lattice_setup(uint32_t Nt, uint32_t Nx) {
+// The lines the reference's lattice set-up prints (lattice.c:57 and
+// calibrate_rect_mult, llr.cpp:155-171 / llr2d.cpp:155-170): the rectangle
+// multiplicity is calibrated as the reference does it, on the reference's
+// own hot start for this seed (the device replay of its set-up), so the
+// value printed is the reference's.  The engine itself counts every 1x2
+// rectangle once, which is that value wherever no rectangle wraps onto
+// itself (extents >= 3).  Returns the calibrated value, or -1 after an error.
+double print_lattice_setup(const char *prog, dsa_group *g, uint32_t D, uint32_t Nt, uint32_t Nx, int32_t iseed) {
   printf("Initializing nearest neighbours [%d:%d]... \n", (int)Nt, (int)Nx);
-  if (Nt >= 3 && Nx >= 3) printf("RECT_MULT calibrated: %.6f\n", 6.0);
+  dsa_llr_ref *R = nullptr;
+  double rm = 0;
+  if (dsa_llr_ref_create(g, D, Nt, Nx, 1, 0.0, iseed, &R) || dsa_llr_ref_rect_mult(R, &rm)) {
+    dsa_llr_ref_destroy(R);
+    die(prog, "RECT_MULT calibration");
+    return -1;
+  }
+  dsa_llr_ref_destroy(R);
+  printf("RECT_MULT calibrated: %.6f\n", rm);
+  return rm;
+}
+
+// llr2d's setup_lattice output (llr2d.cpp:274-290).
+bool lattice_setup_2d(const char *prog, dsa_group *g, uint32_t D, uint32_t Nt, uint32_t Nx, int32_t iseed) {
+  const double rm = print_lattice_setup(prog, g, D, Nt, Nx, iseed);
+  if (rm < 0) return false;
+  if (fabs(rm - 6.0) > 1e-3) printf("WARNING: RECT_MULT=%.3f != 6 -- rectangles wrap; need lattice extent >= 3\n", rm);
+  fflush(stdout);
+  return true;
 }
 
 dsa_llr_params base_params(uint32_t D, uint32_t Nt, uint32_t Nx) {
@@ -185,14 +213,17 @@ int main_llr(int argc, char **argv) {
   printf("NPLAQ: %u\n", V * D * (D - 1) / 2);
   printf("NRECT: %u\n", V * D * (D - 1));
   printf("AXIS: %d\n", axis);
-  print_lattice_setup(Nt, Nx);
-  if (axis == 2 && (Nt < 3 || Nx < 3))
-    printf("WARNING: lattice extent < 3 -- the 1x2 rectangles wrap on this lattice\n"
-           "  (extent too small in some direction); the S2 action is degenerate and the\n"
-           "  S2 window is invalid. Use a lattice with extent >= 3 for axis=2.\n");
   fflush(stdout);
   dsa_group *g = nullptr;
   if (dsa_group_load(gf, &g)) return die(argv[0], "dsa_group_load");
+  const double rect_mult = print_lattice_setup(argv[0], g, D, Nt, Nx, iseed);
+  if (rect_mult < 0) return 2;
+  if (axis == 2 && fabs(rect_mult - 6.0) > 1e-3)
+    printf("WARNING: RECT_MULT=%.3f != 6 -- the 1x2 rectangles wrap on this lattice\n"
+           "  (extent too small in some direction); the S2 action is degenerate and the\n"
+           "  S2 window is invalid. Use a lattice with extent >= 3 for axis=2.\n",
+           rect_mult);
+  fflush(stdout);
   std::vector<std::vector<double>> an(M);
   std::vector<int> reached(2 * R, 0);
   double maxdrift = 0;
@@ -250,6 +281,102 @@ int main_llr(int argc, char **argv) {
     printf("ANE: %8.2f  %9.5f  %9.5f  %lu\n", Ewin, mean, sd, (unsigned long)an[n].size());
   }
   printf("max |incr - recompute| E(axis%d) : %.3e\n", axis, maxdrift);
+  dsa_group_free(g);
+  return 0;
+}
+
+// llr --replay: the reference's own chain (llr.cpp:276-345 at one thread:
+// minstd streams, serial visiting order, per-hit window test, RECT_MULT as
+// calibrated), replayed on the device one window after another, so every line
+// equals the reference's.  A parity tool for small lattices.
+int main_llr_replay(int argc, char **argv) {
+  if (argc < 13) {
+    fprintf(stderr,
+            "usage: %s --replay group D Nt Nx axis bg Etop Ebot step hw a0 seed [K] [NRM] [R]\n"
+            "  axis=1: window S1 (beta1-driven), bg=beta2 ; axis=2: window S2 (beta2-driven), bg=beta1\n",
+            argv[0]);
+    return 1;
+  }
+  const char *gf = argv[1];
+  const uint32_t D = (uint32_t)atoi(argv[2]), Nt = (uint32_t)atoi(argv[3]), Nx = (uint32_t)atoi(argv[4]);
+  const int axis = atoi(argv[5]);
+  const double bg = atof(argv[6]), Etop = atof(argv[7]), Ebot = atof(argv[8]), stepE = atof(argv[9]);
+  const double hw = atof(argv[10]), a0 = atof(argv[11]);
+  const int iseed = atoi(argv[12]);
+  const unsigned K = argc > 13 ? (unsigned)atoi(argv[13]) : 50;
+  const unsigned NRM = argc > 14 ? (unsigned)atoi(argv[14]) : 80;
+  const unsigned R = argc > 15 ? (unsigned)atoi(argv[15]) : 1;
+  const int M = (int)((Etop - Ebot) / stepE + 0.5) + 1;
+  printf("LLR(grp,D,Nt,Nx,axis,bg): %s %d %d %d %d %g | Etop=%g Ebot=%g step=%g hw=%g a0=%g seed=%d K=%u NRM=%u R=%u "
+         "| %d windows\n",
+         gf, (int)D, (int)Nt, (int)Nx, axis, bg, Etop, Ebot, stepE, hw, a0, iseed, K, NRM, R, M);
+  uint32_t V = Nt;
+  for (uint32_t d = 1; d < D; ++d) V *= Nx;
+  printf("NPLAQ: %u\n", V * D * (D - 1) / 2);
+  printf("NRECT: %u\n", V * D * (D - 1));
+  printf("AXIS: %d\n", axis);
+  fflush(stdout);
+  dsa_group *g = nullptr;
+  if (dsa_group_load(gf, &g)) return die(argv[0], "dsa_group_load");
+  dsa_llr_ref *C = nullptr;
+  if (dsa_llr_ref_create(g, D, Nt, Nx, axis, bg, iseed, &C)) return die(argv[0], "dsa_llr_ref_create");
+  double rect_mult = 0;
+  dsa_llr_ref_rect_mult(C, &rect_mult);
+  printf("Initializing nearest neighbours [%d:%d]... \n", (int)Nt, (int)Nx);
+  printf("RECT_MULT calibrated: %.6f\n", rect_mult);
+  if (axis == 2 && fabs(rect_mult - 6.0) > 1e-3)
+    printf("WARNING: RECT_MULT=%.3f != 6 -- the 1x2 rectangles wrap on this lattice\n"
+           "  (extent too small in some direction); the S2 action is degenerate and the\n"
+           "  S2 window is invalid. Use a lattice with extent >= 3 for axis=2.\n",
+           rect_mult);
+  fflush(stdout);
+  std::vector<std::vector<double>> an(M);
+  // run_pass (llr.cpp:256-274)
+  auto run_pass = [&](int dir, int *reached) -> int {
+    *reached = 0;
+    if (dsa_llr_ref_start(C, dir > 0 ? DSA_START_HOT : DSA_START_COLD)) return die(argv[0], "dsa_llr_ref_start");
+    double a_prev = a0;
+    for (int s = 0; s < M; ++s) {
+      const int n = dir > 0 ? s : M - 1 - s;
+      const double Ewin = Etop - n * stepE, Elo = Ewin - hw, Ehi = Ewin + hw;
+      int32_t ok = 0;
+      if (dsa_llr_ref_seed_window(C, Elo, Ehi, a_prev, &ok)) return die(argv[0], "dsa_llr_ref_seed_window");
+      if (!ok) break;
+      double a = 0;
+      if (dsa_llr_ref_rm_solve(C, Ewin, hw, a_prev, K, NRM, &a)) return die(argv[0], "dsa_llr_ref_rm_solve");
+      an[n].push_back(a);
+      a_prev = a;
+      ++*reached;
+    }
+    return 0;
+  };
+  for (unsigned r = 0; r < R; ++r) {
+    if (dsa_llr_ref_reseed(C, r)) return die(argv[0], "dsa_llr_ref_reseed");
+    int rd = 0, ru = 0;
+    if (int rc = run_pass(+1, &rd)) return rc;
+    if (int rc = run_pass(-1, &ru)) return rc;
+    printf("repeat %u: down reached %d/%d, up reached %d/%d\n", r, rd, M, ru, M);
+    fflush(stdout);
+  }
+  printf("\n# Ewin   a_n        sd        nrep\n");
+  for (int n = 0; n < M; ++n) {
+    const double Ewin = Etop - n * stepE;
+    if (an[n].empty()) {
+      printf("ANE: %8.2f   --(unreached)\n", Ewin);
+      continue;
+    }
+    double mean = 0;
+    for (double x : an[n]) mean += x;
+    mean /= an[n].size();
+    double var = 0;
+    for (double x : an[n]) var += (x - mean) * (x - mean);
+    const double sd = an[n].size() > 1 ? sqrt(var / (an[n].size() - 1)) : 0.0;
+    printf("ANE: %8.2f  %9.5f  %9.5f  %lu\n", Ewin, mean, sd, (unsigned long)an[n].size());
+  }
+  double drift = 0;
+  dsa_llr_ref_state(C, nullptr, nullptr, &drift, nullptr, nullptr);
+  printf("max |incr - recompute| E(axis%d) : %.3e\n", axis, drift);
+  dsa_llr_ref_destroy(C);
   dsa_group_free(g);
   return 0;
 }
@@ -432,14 +559,13 @@ int main_llr2d_full(int argc, char **argv) {
   uint32_t V = Nt;
   for (uint32_t d = 1; d < D; ++d) V *= Nx;
   printf("NPLAQ: %u\nNRECT: %u\n", V * D * (D - 1) / 2, V * D * (D - 1));
-  print_lattice_setup(Nt, Nx);
-  if (Nt < 3 || Nx < 3) printf("WARNING: lattice extent < 3 -- rectangles wrap; need lattice extent >= 3\n");
   fflush(stdout);
   dsa_group *g = nullptr;
   if (dsa_group_load(gf, &g)) return die(argv[0], "dsa_group_load");
+  if (!lattice_setup_2d(argv[0], g, D, Nt, Nx, iseed)) return 2;
   std::vector<std::vector<std::vector<double>>> a1s(G.N1, std::vector<std::vector<double>>(G.N2)), a2s = a1s;
   std::vector<int> creach(R, 0), hreach(R, 0);
-  double maxdrift = 0;
+  double drift1 = 0, drift2 = 0;
   {
     Runner run(argv[0], g, D, std::max<uint32_t>(2 * R * (1 + 2 * G.nperp), 1));
     if (!run.ok()) return die(argv[0], "dsa_llr_create");
@@ -454,7 +580,8 @@ int main_llr2d_full(int argc, char **argv) {
       W.start(r, 0, {});
     }
     if (int rc = run.run()) return rc;
-    maxdrift = run.maxdrift();
+    drift1 = run.drift1();
+    drift2 = run.drift2();
   }
   for (unsigned r = 0; r < R; ++r)
     printf("repeat %u: cold ridge %d/%d, hot ridge %d/%d\n", r, creach[r], G.N1, hreach[r], G.N1);
@@ -482,8 +609,7 @@ int main_llr2d_full(int argc, char **argv) {
       printf("ANE2: %9.2f %9.2f %10.6f %10.6f %9.5f %9.5f %d\n", G.E1(i), G.E2(i, j), m1, m2, sd1, sd2, n);
     }
   printf("reached %d/%d cells\n", nreached, G.N1 * G.N2);
-  // the engine keeps one drift figure per chain, max over S1 and S2
-  printf("max |incr-recompute|: S1 %.3e  S2 %.3e\n", maxdrift, maxdrift);
+  printf("max |incr-recompute|: S1 %.3e  S2 %.3e\n", drift1, drift2);
   dsa_group_free(g);
   return 0;
 }
@@ -502,10 +628,10 @@ int main_llr2d_seed(int argc, char **argv) {
   const char *cfgdir = argv[17];
   printf("LLR2D(seed): %s %d %d %d | E1=[%g,%g] step1=%g hw1=%g | ridge c0=%g c1=%g c2=%g step2=%g hw2=%g nperp=%d\n",
          gf, (int)D, (int)Nt, (int)Nx, G.E1top, G.E1bot, G.step1, G.hw1, G.c0, G.c1, G.c2, G.step2, G.hw2, G.nperp);
-  print_lattice_setup(Nt, Nx);
   fflush(stdout);
   dsa_group *g = nullptr;
   if (dsa_group_load(gf, &g)) return die(argv[0], "dsa_group_load");
+  if (!lattice_setup_2d(argv[0], g, D, Nt, Nx, iseed)) return 2;
   std::vector<std::vector<bool>> saved(G.N1, std::vector<bool>(G.N2, false));
   int idx = 0, err = 0;
   {
@@ -552,6 +678,8 @@ int main_llr2d_solve(int argc, char **argv) {
   const double a0 = atof(argv[8]);
   const unsigned K = (unsigned)atoi(argv[9]), NRM = (unsigned)atoi(argv[10]);
   const int iseed = atoi(argv[11]);
+  // (the reference prints its lattice set-up before it finds a manifest
+  // missing, llr2d.cpp:493-495; here the missing manifest is reported first)
   FILE *mf = fopen(manifest, "r");
   if (!mf) {
     fprintf(stderr, "solve: cannot open manifest %s\n", manifest);
@@ -559,7 +687,7 @@ int main_llr2d_solve(int argc, char **argv) {
   }
   dsa_group *g = nullptr;
   if (dsa_group_load(gf, &g)) return die(argv[0], "dsa_group_load");
-  print_lattice_setup(Nt, Nx);
+  if (!lattice_setup_2d(argv[0], g, D, Nt, Nx, iseed)) return 2;
   uint64_t V = Nt;
   for (uint32_t d = 1; d < D; ++d) V *= Nx;
   struct Cell {
@@ -585,7 +713,7 @@ int main_llr2d_solve(int argc, char **argv) {
     cfgs.push_back(std::move(a));
   }
   fclose(mf);
-  double maxdrift = 0;
+  double drift1 = 0, drift2 = 0;
   {
     Runner run(argv[0], g, D, std::max<uint32_t>((uint32_t)cells.size(), 1));
     if (!run.ok()) return die(argv[0], "dsa_llr_create");
@@ -609,11 +737,12 @@ int main_llr2d_solve(int argc, char **argv) {
       run.submit(std::move(t));
     }
     if (int rc = run.run()) return rc;
-    maxdrift = run.maxdrift();
+    drift1 = run.drift1();
+    drift2 = run.drift2();
   }
   for (const Cell &c : cells)
     printf("ANE2: %9.2f %9.2f %10.6f %10.6f %9.5f %9.5f %d\n", c.E1, c.E2, c.res.a, c.res.a2, 0.0, 0.0, 1);
-  fprintf(stderr, "SOLVE[%d,%d): %d cells, drift S1 %.3e S2 %.3e\n", lo, hi, (int)cells.size(), maxdrift, maxdrift);
+  fprintf(stderr, "SOLVE[%d,%d): %d cells, drift S1 %.3e S2 %.3e\n", lo, hi, (int)cells.size(), drift1, drift2);
   dsa_group_free(g);
   return 0;
 }
@@ -627,5 +756,14 @@ int main(int argc, char **argv) {
     if (argc > 1 && strcmp(argv[1], "solve") == 0) return main_llr2d_solve(argc - 1, argv + 1);
     return main_llr2d_full(argc, argv);
   }
+  // llr --replay: the reference's own chain, bit for bit
+  for (int k = 1; k < argc; ++k)
+    if (strcmp(argv[k], "--replay") == 0) {
+      std::vector<char *> rest;
+      for (int j = 0; j < argc; ++j)
+        if (j != k) rest.push_back(argv[j]);
+      rest.push_back(nullptr);
+      return main_llr_replay(argc - 1, rest.data());
+    }
   return main_llr(argc, argv);
 }

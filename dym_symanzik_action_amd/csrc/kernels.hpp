@@ -150,6 +150,7 @@ struct LlrState {
   double a2, sumE2;  // axis 3: tilt of S2 and its sum
   double last_da;    // |delta a| of the last RM step
   double maxdrift;   // max |incremental - recomputed| energy
+  double drift1, drift2;  // the same for S1 and for S2 separately (llr2d.cpp:254-255)
   uint32_t phase, k, m, seed_sweeps, drive, resync, reached, pad;
   uint64_t sweep;    // Philox sweep counter
   unsigned long long acc, hit;
@@ -175,6 +176,43 @@ hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, uint32_t n, u
 hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n, hipStream_t s);
 hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, uint32_t n, const ObsTask *tasks,
                              uint32_t ntasks, hipStream_t s);
+
+// ---- the reference llr's serial chain, replayed (llr_ref.hip) ----
+struct LrefState {
+  double S1, S2;       // running energies (llr.cpp:38-39)
+  double rect_mult;    // RECT_MULT (llr.cpp:135, calibrated :155-171)
+  double maxdrift;     // g_maxdrift
+  double sum;          // sum of Ecur over the current RM iteration's sweeps
+  double a;            // the tilt of rm_solve
+  double cal_m1_old, cal_m1_new;  // calibration scratch
+  unsigned long long hit, acc;
+  uint32_t drive;      // drive-in sweeps of the current seed_window
+  uint32_t cal_b;
+};
+struct LrefArgs {
+  uint16_t *links;
+  uint32_t *ms;        // per-site minstd_rand0 states
+  Tables T;
+  Geom G;
+  int axis;
+  double bg;
+  LrefState *st;
+  double *t1, *t2;     // terms of recompute_S1 (V D(D-1)/2) and sum_m1 (V D 6(D-1))
+};
+// update_constrained (driveIn sweeps skipped once inside [Elo, Ehi] or after
+// cap of them; a NaN tilt takes st->a)
+hipError_t launch_lref_update(const LrefArgs &A, double a_tilt, double Elo, double Ehi, int driveIn, uint32_t cap,
+                              hipStream_t s);
+// sums[0] = recompute_S1, sums[1] = sum_m1, each in the reference's serial order
+hipError_t launch_lref_sums(const LrefArgs &A, double *sums, hipStream_t s);
+// mode 0: resync_E; mode 1: the end of Robbins-Monro iteration m (a, drift, resync)
+hipError_t launch_lref_resync(const LrefArgs &A, const double *sums, int mode, double Ewin, double W, uint32_t m,
+                              uint32_t K, hipStream_t s);
+hipError_t launch_lref_hot(const LrefArgs &A, hipStream_t s);
+hipError_t launch_lref_seed(const LrefArgs &A, uint32_t base, hipStream_t s);
+hipError_t launch_lref_calibrate(const LrefArgs &A, const double *sums_old, const double *sums_new, int step,
+                                 hipStream_t s);
+hipError_t launch_lref_set(const LrefArgs &A, double a, uint32_t drive, hipStream_t s);
 
 // `block` lanes per block (a multiple of 64, at most kBatchBlockMax); every
 // task holds at most `block` links.

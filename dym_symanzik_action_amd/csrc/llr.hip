@@ -707,6 +707,8 @@ __global__ void k_llr_resync_apply(const LlrChain *chains, LlrState *st, uint32_
     const double d1 = fabs(from_fix(S.S1) - from_fix(S.r1)), d2 = fabs(from_fix(S.S2) - from_fix(S.r2));
     const double drift = axis == 1 ? d1 : axis == 2 ? d2 : fmax(d1, d2);
     if (drift > S.maxdrift) S.maxdrift = drift;
+    if (d1 > S.drift1) S.drift1 = d1;
+    if (d2 > S.drift2) S.drift2 = d2;
   }
   S.S1 = S.r1;
   S.S2 = S.r2;

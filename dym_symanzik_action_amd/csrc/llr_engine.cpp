@@ -397,6 +397,8 @@ int dsa_llr_status_get(dsa_llr *L, uint32_t slot, dsa_llr_status *out) {
   out->S2 = dsa::from_fix(s.S2);
   out->last_da = s.last_da;
   out->maxdrift = s.maxdrift;
+  out->drift1 = s.drift1;
+  out->drift2 = s.drift2;
   out->phase = s.phase;
   out->iterations = s.m;
   out->seed_sweeps = s.seed_sweeps;

@@ -490,7 +490,7 @@ class LlrStatus(C.Structure):
         ("a", C.c_double), ("S1", C.c_double), ("S2", C.c_double), ("last_da", C.c_double),
         ("maxdrift", C.c_double), ("phase", C.c_uint32), ("iterations", C.c_uint32),
         ("seed_sweeps", C.c_uint32), ("reached", C.c_uint32), ("sweeps", C.c_uint64), ("hit", C.c_uint64),
-        ("acc", C.c_uint64), ("a2", C.c_double),
+        ("acc", C.c_uint64), ("a2", C.c_double), ("drift1", C.c_double), ("drift2", C.c_double),
     ]
 
     def as_dict(self):
@@ -597,6 +597,86 @@ class LlrEngine:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().dsa_llr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class LlrRef:
+    """The reference llr's serial chain replayed on the device
+    (dsa_llr_ref_*): main's set-up (llr.cpp:296-321, RECT_MULT calibrated)
+    at creation, then run_pass's steps (llr.cpp:223-274) one call at a time,
+    bit for bit as the reference runs them at one thread."""
+
+    def __init__(self, group: Group, D: int, Nt: int, Nx: int, axis: int, bg: float, seed: int):
+        L = lib()
+        if not hasattr(L.dsa_llr_ref_create, "_sig"):
+            vp, u32, i32, dbl = C.c_void_p, C.c_uint32, C.c_int32, C.c_double
+            L.dsa_llr_ref_create.argtypes = [vp, u32, u32, u32, i32, dbl, i32, C.POINTER(vp)]
+            L.dsa_llr_ref_destroy.argtypes = [vp]
+            L.dsa_llr_ref_destroy.restype = None
+            L.dsa_llr_ref_rect_mult.argtypes = [vp, C.POINTER(dbl)]
+            L.dsa_llr_ref_reseed.argtypes = [vp, u32]
+            L.dsa_llr_ref_start.argtypes = [vp, i32]
+            L.dsa_llr_ref_seed_window.argtypes = [vp, dbl, dbl, dbl, C.POINTER(i32)]
+            L.dsa_llr_ref_rm_solve.argtypes = [vp, dbl, dbl, dbl, u32, u32, C.POINTER(dbl)]
+            L.dsa_llr_ref_state.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl), C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_uint64)]
+            L.dsa_llr_ref_get_links.argtypes = [vp, vp]
+            L.dsa_llr_ref_create._sig = True
+        h = C.c_void_p()
+        _check(L.dsa_llr_ref_create(group._h, D, Nt, Nx, axis, bg, seed, C.byref(h)))
+        self._h = h
+        self.group = group
+        self.V, self.D = Nt * Nx ** (D - 1), D
+
+    @property
+    def rect_mult(self) -> float:
+        v = C.c_double()
+        _check(lib().dsa_llr_ref_rect_mult(self._h, C.byref(v)))
+        return v.value
+
+    def reseed(self, repeat: int) -> None:
+        _check(lib().dsa_llr_ref_reseed(self._h, repeat))
+
+    def start(self, hot: bool) -> None:
+        _check(lib().dsa_llr_ref_start(self._h, DSA_START["hot" if hot else "cold"]))
+
+    def seed_window(self, Elo: float, Ehi: float, a_guess: float) -> bool:
+        r = C.c_int32()
+        _check(lib().dsa_llr_ref_seed_window(self._h, Elo, Ehi, a_guess, C.byref(r)))
+        return bool(r.value)
+
+    def rm_solve(self, Ewin: float, hw: float, a0: float, K: int, NRM: int) -> float:
+        a = C.c_double()
+        _check(lib().dsa_llr_ref_rm_solve(self._h, Ewin, hw, a0, K, NRM, C.byref(a)))
+        return a.value
+
+    def state(self) -> dict:
+        s1, s2, md = C.c_double(), C.c_double(), C.c_double()
+        hit, acc = C.c_uint64(), C.c_uint64()
+        _check(lib().dsa_llr_ref_state(self._h, C.byref(s1), C.byref(s2), C.byref(md), C.byref(hit), C.byref(acc)))
+        return {"S1": s1.value, "S2": s2.value, "maxdrift": md.value, "hit": hit.value, "acc": acc.value}
+
+    @property
+    def a(self) -> np.ndarray:
+        out = np.empty(self.V * self.D, np.int32)
+        _check(lib().dsa_llr_ref_get_links(self._h, _ptr(out)))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().dsa_llr_ref_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -267,6 +267,8 @@ typedef struct {
   uint32_t iterations, seed_sweeps, reached;
   uint64_t sweeps, hit, acc;
   double a2;          /* axis 3: the S2 tilt */
+  double drift1, drift2;  /* max |incremental - recomputed| of S1 and of S2
+                             (llr2d.cpp:254-255 tracks both) */
 } dsa_llr_status;
 
 typedef struct dsa_llr dsa_llr;
@@ -303,6 +305,40 @@ int dsa_llr_get_links(dsa_llr *L, uint32_t slot, int32_t *a);
  * serves windows one process at a time: scripts/llr2d_parallel.sh:28-37.) */
 int dsa_llr_serve(dsa_llr *L, const dsa_group *const *groups, const dsa_llr_params *req, const double *arrival_s,
                   uint32_t nreq, uint32_t chunk, double *admit_s, double *done_s, dsa_llr_status *status);
+
+/* ---- the reference llr's serial chain, replayed on the device ---------------
+ * Replaces llr.cpp's own chain (llr.cpp:178-274, 296-326) exactly as the
+ * reference runs it at one thread, for bit-exact replay: per-site
+ * std::minstd_rand0 streams seeded iseed + i (unsigned), hot links from site
+ * 0's stream through uniform_int_distribution(0, P-1), RECT_MULT calibrated
+ * from one link flip (calibrate_rect_mult), links visited serially (d, then
+ * i) with the window tested after every hit on running double energies,
+ * full-lattice energies recomputed in the reference's serial order.  Links,
+ * energies, RECT_MULT and a_n equal the reference's bit for bit (the one
+ * deviation possible is glibc's exp against this library's, within 1 ulp).
+ * One lane walks the lattice, so this is a parity and replay path for small
+ * lattices; dsa_llr_* is the throughput path.  axis 1 or 2 (llr.cpp AXIS),
+ * bg its BG.  Create performs main's set-up (llr.cpp:314-320). */
+typedef struct dsa_llr_ref dsa_llr_ref;
+int dsa_llr_ref_create(const dsa_group *g, uint32_t D, uint32_t Nt, uint32_t Nx, int32_t axis, double bg,
+                       int32_t seed, dsa_llr_ref **out);
+void dsa_llr_ref_destroy(dsa_llr_ref *R);
+/* RECT_MULT as calibrate_rect_mult found it (llr.cpp:155-171). */
+int dsa_llr_ref_rect_mult(dsa_llr_ref *R, double *rect_mult);
+/* rnd[i].seed(iseed + repeat*1000003 + i) (llr.cpp:326). */
+int dsa_llr_ref_reseed(dsa_llr_ref *R, uint32_t repeat);
+/* run_pass's start (llr.cpp:258-260): DSA_START_HOT draws every link from
+ * site 0's stream, DSA_START_COLD sets the identity; then resync_E. */
+int dsa_llr_ref_start(dsa_llr_ref *R, int32_t start);
+/* seed_window (llr.cpp:223-231): greedy drive-in sweeps until the running
+ * windowed energy is in [Elo, Ehi] or 20000 sweeps, then resync_E. */
+int dsa_llr_ref_seed_window(dsa_llr_ref *R, double Elo, double Ehi, double a_guess, int32_t *reached);
+/* rm_solve (llr.cpp:234-250): NRM iterations of K constrained sweeps. */
+int dsa_llr_ref_rm_solve(dsa_llr_ref *R, double Ewin, double hw, double a0, uint32_t K, uint32_t NRM, double *a);
+/* Running energies S1, S2 (llr.cpp:38-39), g_maxdrift and the hit/acc totals;
+ * any pointer may be NULL. */
+int dsa_llr_ref_state(dsa_llr_ref *R, double *S1, double *S2, double *maxdrift, uint64_t *hit, uint64_t *acc);
+int dsa_llr_ref_get_links(dsa_llr_ref *R, int32_t *a);
 
 /* Measurement helpers (no counterpart in the reference, which times with
  * gettimeofday around update(), timer.cpp:11-27).  Mode 1: every pass launch

@@ -137,13 +137,15 @@ void ogp_llr_set_links(void *Lv, const int32_t *in) {
   memcpy(L->c.L.a, in, (size_t)L->c.L.V * L->c.L.D * 4);
   L->resync = 2;
 }
-/* out: a, S1, S2, last_da, maxdrift, phase, m, seed_sweeps, reached, sweep, hit, acc, a2 */
+/* out: a, S1, S2, last_da, maxdrift, phase, m, seed_sweeps, reached, sweep, hit, acc, a2, drift1, drift2 */
 void ogp_llr_status(void *Lv, double *out) {
   ogl_dev *L = (ogl_dev *)Lv;
   out[0] = L->a; out[1] = (double)L->S1 / 4294967296.0; out[2] = (double)L->S2 / 4294967296.0;
   out[3] = L->last_da; out[4] = L->maxdrift; out[5] = L->phase; out[6] = L->m; out[7] = L->seed_sweeps;
   out[8] = L->reached; out[9] = (double)L->sweep; out[10] = (double)L->hit; out[11] = (double)L->acc;
   out[12] = L->a2;
+  out[13] = L->drift1;
+  out[14] = L->drift2;
 }
 void ogp_llr_links(void *Lv, int32_t *out) {
   ogl_dev *L = (ogl_dev *)Lv;
@@ -195,4 +197,8 @@ void ogp_lref_state(void *Rv, double *out) {
 void ogp_lref_links(void *Rv, int32_t *out) {
   ogl_ref *R = (ogl_ref *)Rv;
   memcpy(out, R->c.L.a, (size_t)R->c.L.V * R->c.L.D * 4);
+}
+/* one update_constrained sweep (llr.cpp:178-219) at tilt a */
+void ogp_lref_sweep(void *Rv, double a, double Elo, double Ehi, int drive) {
+  ogl_ref_update_constrained((ogl_ref *)Rv, a, Elo, Ehi, drive);
 }

@@ -295,6 +295,8 @@ static void dev_resync(ogl_dev *L) {
     const double d1 = fabs(ogl_unfix(L->S1) - ogl_unfix(r1)), d2 = fabs(ogl_unfix(L->S2) - ogl_unfix(r2));
     const double drift = L->axis == 1 ? d1 : L->axis == 2 ? d2 : fmax(d1, d2);
     if (drift > L->maxdrift) L->maxdrift = drift;
+    if (d1 > L->drift1) L->drift1 = d1;
+    if (d2 > L->drift2) L->drift2 = d2;
   }
   L->S1 = r1;
   L->S2 = r2;

@@ -50,6 +50,7 @@ typedef struct {
   double tol;
   long long S1, S2;  /* fixed point, units 2^-32 */
   double a, sumE, last_da, maxdrift;
+  double drift1, drift2; /* the drift of S1 and of S2 separately */
   double a2, sumE2;  /* axis 3: tilt of S2 and its sum */
   unsigned phase, k, m, seed_sweeps, reached, resync;
   uint64_t sweep, acc, hit;

@@ -236,10 +236,10 @@ class OLlr:
         lib().ogp_llr_step(self.h, n)
 
     def status(self):
-        out = np.empty(13, np.float64)
+        out = np.empty(15, np.float64)
         lib().ogp_llr_status(self.h, _p(out))
         keys = ["a", "S1", "S2", "last_da", "maxdrift", "phase", "iterations", "seed_sweeps", "reached", "sweeps",
-                "hit", "acc", "a2"]
+                "hit", "acc", "a2", "drift1", "drift2"]
         return dict(zip(keys, out.tolist()))
 
     @property
@@ -276,6 +276,7 @@ class OLlrRef:
             L.ogp_lref_rm_solve.argtypes = [vp, dbl, dbl, dbl, u32, u32]
             L.ogp_lref_state.argtypes = [vp, vp]
             L.ogp_lref_links.argtypes = [vp, vp]
+            L.ogp_lref_sweep.argtypes = [vp, dbl, dbl, dbl, i32]
             L.ogp_lref_new._sig = True
         self.g = g
         self.h = L.ogp_lref_new(g.h, D, Nt, Nx, axis, bg, seed)
@@ -298,6 +299,10 @@ class OLlrRef:
 
     def rm_solve(self, Ewin, hw, a0, K, NRM):
         return lib().ogp_lref_rm_solve(self.h, Ewin, hw, a0, K, NRM)
+
+    def sweep(self, a, Elo, Ehi, drive=False):
+        """One update_constrained call (llr.cpp:178-219) at tilt a."""
+        lib().ogp_lref_sweep(self.h, a, Elo, Ehi, 1 if drive else 0)
 
     def state(self):
         out = np.empty(5, np.float64)

@@ -333,3 +333,34 @@ def test_llr2d_solve_skips_missing_cells(tmp_path):
     assert "solve: missing cfg cfg/" in err and "SOLVE[0,3): 2 cells" in err
     assert [l.split()[1:3] for l in out.splitlines() if l.startswith("ANE2:")] == \
         [l.split()[4:6] for l in man[1:]]
+
+
+@pytest.mark.parametrize("fixture", ["llr_axis1_3x3.txt", "llr_axis2_4x3.txt", "llr_s216_axis1.txt"])
+def test_llr_replay_prints_reference_output(fixture):
+    """`llr --replay` (the reference's serial chain replayed on the device)
+    prints the reference binary's stdout line for line: the header lines,
+    RECT_MULT as calibrated, the repeat lines, every ANE line and the drift
+    line (tests/golden/llr_*.txt, the reference llr at one thread)."""
+    ref = open(os.path.join(GOLDEN, fixture)).read().splitlines()
+    args = ref[0].split()[2:-1]
+    say("case llr --replay", fixture, args)
+    out, _ = run(LLR, ["--replay", group_file(args[0])] + args[1:], trace=False)
+    strip = lambda l: l.replace(group_file(args[0]), args[0]).replace("/root/reference/groups/" + args[0], args[0])
+    assert [strip(l) for l in out.splitlines()] == [strip(l) for l in ref[1:]]
+
+
+def test_llr_rect_mult_line_is_the_references_calibration():
+    """On lattices with an extent of 2 the rectangles wrap and the reference's
+    calibrated RECT_MULT differs from 6 (llr.cpp:155-171, 318-321): the
+    default drop-in prints the reference's value and its warning for axis 2
+    (the value of tests/golden/llrladder_axis2_3x2.npz, calibrated by the
+    reference's own calibrate_rect_mult on its hot start)."""
+    f = np.load(os.path.join(GOLDEN, "llrladder_axis2_3x2.npz"))
+    args = [str(x) for x in f["args"]]
+    say("case llr RECT_MULT line", args)
+    proc = subprocess.run([LLR, group_file(args[0])] + args[1:4] + ["2", "0.8", "-50", "-50", "50", "25", "0.5",
+                                                                      args[11], "1", "1"],
+                          capture_output=True, text=True, timeout=300)
+    lines = proc.stdout.splitlines()
+    assert f"RECT_MULT calibrated: {float(f['rect_mult']):.6f}" in lines
+    assert f"WARNING: RECT_MULT={float(f['rect_mult']):.3f} != 6 -- the 1x2 rectangles wrap on this lattice" in lines

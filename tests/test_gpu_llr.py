@@ -12,8 +12,8 @@ from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-KEYS = ["a", "S1", "S2", "last_da", "maxdrift", "phase", "iterations", "seed_sweeps", "reached", "sweeps", "hit",
-        "acc"]
+KEYS = ["a", "S1", "S2", "last_da", "maxdrift", "drift1", "drift2", "phase", "iterations", "seed_sweeps", "reached",
+        "sweeps", "hit", "acc"]
 
 # group, D, Nt, Nx, axis, bg, Ewin, hw, a0, seed, start, K, NRM
 CASES = [
