@@ -293,8 +293,8 @@ def main():
                     help="sweeps per chain of the one-chain-at-a-time comparison")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="LLR window requests for the serving measurement (0: skip)")
-    ap.add_argument("--serve-slots", type=int, default=16)
-    ap.add_argument("--serve-budget-mb", type=int, default=64,
+    ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--serve-budget-mb", type=int, default=128,
                     help="HBM budget of the serving engine in MiB (admission by footprint)")
     ap.add_argument("--serve-serial-every", type=int, default=5,
                     help="time every n-th request alone for the one-at-a-time rate")

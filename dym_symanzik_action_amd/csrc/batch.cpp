@@ -71,6 +71,11 @@ struct dsa_batch {
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
   uint32_t graph_sweeps = 0;
+  // per chain: the descriptors as uploaded and the accepted hits seen so
+  // far, for choosing each class chain's plaquette-first test from its
+  // acceptance over the previous call (dsa::kBoundAcceptance)
+  std::vector<dsa::BatchChain> desc;
+  std::vector<unsigned long long> acc_seen;
 };
 
 namespace {
@@ -170,6 +175,30 @@ int tune_tasks(dsa_batch *b, size_t nlinks, size_t nms) {
   }
   release();
   return set_tasks(b, best);
+}
+
+// After a call of n sweeps: every class chain whose acceptance over the call
+// was below dsa::kBoundAcceptance runs the plaquette-first test in the next
+// call (descriptors re-uploaded when a choice changes; the stream is idle).
+int choose_bounds(dsa_batch *b, uint32_t n) {
+  if (!n) return 0;
+  const size_t nc = b->chains.size();
+  std::vector<unsigned long long> a(nc);
+  HIP_TRY(hipMemcpy(a.data(), b->d_acc, nc * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  bool changed = false;
+  for (size_t c = 0; c < nc; ++c) {
+    const HostChain &hc = b->chains[c];
+    const double hits = (double)n * hc.G.V * hc.G.D * dsa::kNHit;
+    const int want = hc.cls && (double)(a[c] - b->acc_seen[c]) < dsa::kBoundAcceptance * hits ? 1 : 0;
+    b->acc_seen[c] = a[c];
+    if (want != b->desc[c].bound) {
+      b->desc[c].bound = want;
+      changed = true;
+    }
+  }
+  if (changed)
+    HIP_TRY(hipMemcpy(b->d_chains, b->desc.data(), nc * sizeof(dsa::BatchChain), hipMemcpyHostToDevice));
+  return 0;
 }
 
 int enqueue_measure(dsa_batch *b) {
@@ -275,6 +304,7 @@ int dsa_batch_finalize(dsa_batch *b) {
     d.cls = hc.cls ? 1 : 0;
     d.cmargin = hc.margin * b->margin_scale;
     d.fall = b->d_fall;
+    d.bound = 0;
     HIP_TRY(hipMemcpy(b->d_sites + hc.sites_off, hc.S.sites.data(), hc.S.sites.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
     // start: the same kernels and host draws as a lone chain
@@ -309,6 +339,8 @@ int dsa_batch_finalize(dsa_batch *b) {
   HIP_TRY(hipMalloc(&b->d_partials, 4 * obs.size() * sizeof(double)));
   HIP_TRY(hipMalloc(&b->d_out, 4 * nc * sizeof(double)));
   HIP_TRY(hipMemcpy(b->d_chains, desc.data(), nc * sizeof(dsa::BatchChain), hipMemcpyHostToDevice));
+  b->desc = desc;
+  b->acc_seen.assign(nc, 0);
   HIP_TRY(hipMemcpy(b->d_obs, obs.data(), obs.size() * sizeof(dsa::ObsTask), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(b->d_obs_first, ofirst.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(b->d_obs_n, on.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -391,7 +423,7 @@ int dsa_batch_sweep(dsa_batch *b, uint32_t n) {
   HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_ms = ms;
   b->sweeps += n;
-  return 0;
+  return choose_bounds(b, n);
 }
 
 int dsa_batch_measure(dsa_batch *b, double *out) {
@@ -436,7 +468,7 @@ int dsa_batch_run_cycles(dsa_batch *b, uint32_t ncycles, uint32_t sweeps_per_cyc
   HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_ms = ms;
   b->sweeps += (uint64_t)ncycles * sweeps_per_cycle;
-  return 0;
+  return choose_bounds(b, ncycles * sweeps_per_cycle);
 }
 
 int dsa_batch_info(dsa_batch *b, uint32_t *nchains, uint32_t *npasses, uint32_t *nblocks_per_sweep) {

@@ -57,6 +57,10 @@ struct dsa_chain {
   size_t out_cap = 0;
   uint64_t sweep = 0, hit = 0;
   double last_ms = 0;
+  // the narrow class form's plaquette-first test for the next call, chosen
+  // from the acceptance of the previous one (dsa::kBoundAcceptance)
+  bool bound_next = false;
+  unsigned long long acc_host = 0;
   int max_blocks = 2048;
   // sweep kernel: DSA_KERNEL_*, the class-kernel forms that can run here
   // (dsa::kCls*, kClsNone if neither), their decision margins
@@ -89,6 +93,7 @@ dsa::Tables dsa::tables_view(const dsa_group *g, const DeviceTables &t) {
   T.cls = t.cls;
   T.cprod = t.cprod;
   T.rcq = t.rcq;
+  T.mdq = t.mdq;
   T.ncls = (uint32_t)g->g.rc.size();
   T.cval = t.cval;
   T.cval0 = t.cval0;
@@ -217,7 +222,7 @@ namespace {
 
 void free_tables(DeviceTables &t) {
   for (void *p : {(void *)t.mult, (void *)t.inv, (void *)t.pool, (void *)t.retr, (void *)t.imtr, (void *)t.cls,
-                  (void *)t.cprod, (void *)t.rcq, (void *)t.cval, (void *)t.cval0, (void *)t.mat})
+                  (void *)t.cprod, (void *)t.rcq, (void *)t.mdq, (void *)t.cval, (void *)t.cval0, (void *)t.mat})
     if (p) (void)hipFree(p);
   t = DeviceTables();
 }
@@ -246,6 +251,10 @@ int fill_tables(const dsa::Group &G, dsa_group *g, DeviceTables &t) {
     HIP_TRY(hipMemcpy(t.cls, G.cls.data(), P, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.cprod, G.cprod.data(), G.cprod.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.rcq, G.rcq.data(), G.rcq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!G.mdq.empty()) {
+      HIP_TRY(hipMalloc(&t.mdq, G.mdq.size() * sizeof(int32_t)));
+      HIP_TRY(hipMemcpy(t.mdq, G.mdq.data(), G.mdq.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
   }
   if (!G.cval.empty()) {
     const size_t nb = dsa::pad16(G.cval.size() * sizeof(int16_t)), nb0 = dsa::pad16(P * sizeof(int16_t));
@@ -514,6 +523,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   const int mode = cls_mode(c);
   A.cmargin = (mode == dsa::kClsMatrix ? c->mat_m0 : mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
   A.cmargin_acc = c->mat_macc * c->margin_scale;
+  A.bound = c->bound_next ? 1 : 0;
   const size_t np = c->S.passes.size();
   if (c->profiling == 1 && c->pev.size() < 2 * np) {
     for (size_t q = c->pev.size(); q < 2 * np; ++q) {
@@ -548,11 +558,17 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
     }
     c->sweep++;
   }
+  const unsigned long long acc_before = c->acc_host;
+  HIP_TRY(hipMemcpyAsync(&c->acc_host, c->d_acc, sizeof c->acc_host, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev1));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  if (n) {
+    const double hits = (double)n * (double)c->G.V * c->G.D * dsa::kNHit;
+    c->bound_next = (double)(c->acc_host - acc_before) < dsa::kBoundAcceptance * hits;
+  }
   if (c->profiling == 2) {
     c->kern_ms += ms;
     c->kern_launches += (uint64_t)n * np;

@@ -321,6 +321,17 @@ __device__ inline int metro_certified(double dc, double M, double u2) {
   return dec;
 }
 
+// A certain rejection from a lower bound: the exact action change is at least
+// dl - M.  True when exp(-(dl - M)) <= u2 for sure, by the float tier of
+// metro_certified (the same slack); false leaves the hit to the full test.
+__device__ inline bool metro_rejects(double dl, double M, double u2) {
+  if (dl - M <= 0.0) return false;
+  if (dl - M > 37.0 && u2 >= 1e-15) return true;
+  if (dl >= 60.0) return false;
+  const double e = (double)__builtin_amdgcn_exp2f((float)(-dl * 1.4426950408889634));
+  return u2 >= e * (1.0 + (1.02 * M + 0x1p-16));
+}
+
 // The reference's decision of one hit (dym-mod-metro.cpp:176-188) from
 // element values, for proposal b -> b p given t_k = s_k b: s_k = t_k b^-1,
 // so ReTr(b s_k) = retr[b t_k b^-1] and ReTr(b' s_k) = retr[b' t_k b^-1],
@@ -366,12 +377,22 @@ struct ClsNarrow {
   const uint8_t *s_cprod;  // LDS, npool x P: 4 x class of pool[j] * t
   const uint8_t *s_cls;    // LDS, P: 4 x class
   const int32_t *s_rcq;    // LDS (static array), class ReTr in units of 2^-24
+  // LDS: per class, the most one hit can change a term whose product is in
+  // that class (Group::mdq, units of 2^-24); read by the BOUND update only
+  const int32_t *s_md = nullptr;
   static constexpr double kScale = 0x1p-24;
+  static constexpr bool kBound = true;
   using Row = const uint8_t *;
   __device__ Row row(uint32_t jp, uint32_t P) const { return s_cprod + jp * P; }
   __device__ Row row0() const { return s_cls; }
   __device__ int32_t val(Row r, uint32_t t) const {
     return *reinterpret_cast<const int32_t *>(reinterpret_cast<const uint8_t *>(s_rcq) + r[t]);
+  }
+  // the class value of t and its bound, from one class-index read
+  __device__ int32_t val_md(uint32_t t, int32_t &md) const {
+    const uint32_t off = s_cls[t];
+    md = *reinterpret_cast<const int32_t *>(reinterpret_cast<const uint8_t *>(s_md) + off);
+    return *reinterpret_cast<const int32_t *>(reinterpret_cast<const uint8_t *>(s_rcq) + off);
   }
 };
 
@@ -379,6 +400,7 @@ struct ClsWide {
   const int16_t *s_cval;   // LDS, npool x P: class value of pool[j] * t in units of 2^-13
   const int16_t *s_cval0;  // LDS, P: class value of t
   static constexpr double kScale = 0x1p-13;
+  static constexpr bool kBound = false;
   using Row = const int16_t *;
   __device__ Row row(uint32_t jp, uint32_t P) const { return s_cval + jp * P; }
   __device__ Row row0() const { return s_cval0; }
@@ -410,6 +432,7 @@ struct ClsLink {
   uint32_t i, b, b0, ms;
   uint32_t t[NL];
   int32_t r1, m1;
+  int32_t rb;  // sum over the rectangle terms of their classes' bounds (2^-24; 0 without bounds)
 };
 
 // Class sums of the plaquette (k < NS) and rectangle terms of t through `row`.
@@ -426,7 +449,44 @@ __device__ inline void cls_sums(const L &Lk, typename L::Row row, const uint32_t
   m = c;
 }
 
-template <int D, class L>
+// Class sums of t through row 0 (r1, m1) and, with BOUND (the narrow form,
+// its bounds in LDS), the rectangle terms' bound sum rb.
+template <int D, bool BOUND, class L>
+__device__ inline void cls_sums_bound(const L &Lk, ClsLink<D> &S) {
+  constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
+  if constexpr (BOUND && L::kBound) {
+    int32_t a = 0, c = 0, e = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      int32_t md;
+      const int32_t v = Lk.val_md(S.t[k], md);
+      if (k < NS) {
+        a += v;
+      } else {
+        c += v;
+        e += md;
+      }
+    }
+    S.r1 = a;
+    S.m1 = c;
+    S.rb = e;
+  } else {
+    cls_sums<D>(Lk, Lk.row0(), S.t, S.r1, S.m1);
+    S.rb = 0;
+  }
+}
+
+// Class sum of the plaquette terms (k < NS) or of the rectangle terms of t.
+template <int D, bool PLAQ, class L>
+__device__ inline int32_t cls_part(const L &Lk, typename L::Row row, const uint32_t *t) {
+  constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
+  int32_t a = 0;
+#pragma unroll
+  for (int k = PLAQ ? 0 : NS; k < (PLAQ ? NS : NL); ++k) a += Lk.val(row, t[k]);
+  return a;
+}
+
+template <int D, bool BOUND, class L>
 __device__ inline void cls_prepare(ClsLink<D> &S, const LinkCtx &C, const L &Lk, const Geom &G, const TabView &T,
                                    int mu, uint32_t i) {
   S.i = i;
@@ -434,16 +494,24 @@ __device__ inline void cls_prepare(ClsLink<D> &S, const LinkCtx &C, const L &Lk,
   S.lp = (size_t)mu * G.V + lpos(G, i, S.s.par);
   S.b = S.b0 = C.links[S.lp];
   build_t<D>(C.links, G, T, S.s, mu, S.b, S.t);
-  cls_sums<D>(Lk, Lk.row0(), S.t, S.r1, S.m1);
+  cls_sums_bound<D, BOUND>(Lk, S);
   S.ms = C.rng == kRngMinstd ? C.mstate[i] : 0u;
 }
 
-template <int D, class L>
+// BOUND (the narrow form only): each hit is first tested on its plaquette
+// terms with the rectangle terms bounded by |beta2| rb; a hit rejected even
+// at the least action change that allows is rejected without the rectangle
+// lookups.  It pays only where nearly every lane of a wavefront is rejected
+// that way (a wavefront runs the rectangle terms if any lane needs them),
+// which is why it is a separate instantiation chosen at very low acceptance.
+template <int D, bool BOUND, class L>
 __device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk, const ClsCtx &K, const TabView &T,
                                     int mu, uint32_t &nfall, const PreDraws *pre) {
   constexpr int NL = ClsLink<D>::NL;
+  constexpr bool bounded = BOUND && L::kBound;
   const uint32_t link = S.i * (uint32_t)D + (uint32_t)mu;
   const double b1q = C.beta1 * L::kScale, b2q = C.beta2 * L::kScale, margin = K.margin;
+  const double ab2q = fabs(C.beta2) * L::kScale;
   uint32_t acc = 0;
   U4 w{0, 0, 0, 0};
   for (int h = 0; h < kNHit; ++h) {
@@ -460,8 +528,20 @@ __device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk
       jp = (uint32_t)(C.npool_d * minstd_canonical(S.ms));
       u2 = minstd_canonical(S.ms);
     }
+    const typename L::Row row = Lk.row(jp, T.P);
     int32_t r1n, m1n;
-    cls_sums<D>(Lk, Lk.row(jp, T.P), S.t, r1n, m1n);
+    if constexpr (bounded) {
+      // plaquette terms first: the rectangle terms change the action by at
+      // most |beta2| rb, so the exact change is at least dp - bx - M; a hit
+      // rejected even at that change is rejected without the rectangle terms
+      r1n = cls_part<D, true>(Lk, row, S.t);
+      const double dp = b1q * ((double)r1n - (double)S.r1), bx = ab2q * (double)S.rb;
+      const double dl = dp - bx, Ml = margin + 0x1p-50 * (fabs(dp) + bx);
+      if (metro_rejects(dl, Ml, u2)) continue;
+      m1n = cls_part<D, false>(Lk, row, S.t);
+    } else {
+      cls_sums<D>(Lk, row, S.t, r1n, m1n);
+    }
     // class-valued change of action; the exact one lies within M of it
     const double dc = b1q * ((double)r1n - (double)S.r1) + b2q * ((double)m1n - (double)S.m1);
     const double M = margin + 0x1p-50 * fabs(dc);
@@ -475,8 +555,12 @@ __device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk
       S.b = T.M(S.b, p);
 #pragma unroll
       for (int k = 0; k < NL; ++k) S.t[k] = T.M(S.t[k], p);
-      S.r1 = r1n;
-      S.m1 = m1n;
+      if constexpr (bounded) {
+        cls_sums_bound<D, true>(Lk, S);  // the new products' classes and bounds
+      } else {
+        S.r1 = r1n;
+        S.m1 = m1n;
+      }
       ++acc;
     }
   }
@@ -485,13 +569,13 @@ __device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk
   return acc;
 }
 
-template <int D, class L>
+template <int D, bool BOUND = false, class L>
 __device__ inline uint32_t update_link_cls(const LinkCtx &C, const L &Lk, const ClsCtx &K, const Geom &G,
                                            const TabView &T, int mu, uint32_t i, uint32_t &nfall,
                                            const PreDraws *pre = nullptr) {
   ClsLink<D> S;
-  cls_prepare<D>(S, C, Lk, G, T, mu, i);
-  return cls_hits<D>(S, C, Lk, K, T, mu, nfall, pre);
+  cls_prepare<D, BOUND>(S, C, Lk, G, T, mu, i);
+  return cls_hits<D, BOUND>(S, C, Lk, K, T, mu, nfall, pre);
 }
 
 // Matrix form of the hit loop.  rho(g) is the SU(3) matrix the group file
@@ -714,20 +798,25 @@ struct ClsNarrowStage {
   Staged<1, uint16_t> pool;
   Staged<3, uint8_t> cls;
   Staged<1, int32_t> rcq;
+  Staged<1, int32_t> mdq;
   Staged<10, u32x4> tab;
+  // s_md (64 entries) receives the per-class hit bounds where the group has them
   __device__ inline void run(const Tables &Tb, uint16_t *s_inv, uint16_t *s_pool, uint8_t *s_cls, int32_t *s_rcq,
-                             uint8_t *s_tab) {
+                             uint8_t *s_tab, int32_t *s_md = nullptr) {
     const uint32_t n16 = (uint32_t)(pad16((size_t)Tb.npool * Tb.P) / 16);
     const u32x4 *src = reinterpret_cast<const u32x4 *>(Tb.cprod);
+    const uint32_t nmd = s_md && Tb.mdq ? Tb.ncls : 0u;
     inv.load(Tb.inv, Tb.P);
     pool.load(Tb.pool, Tb.npool);
     cls.load(Tb.cls, Tb.P);
     rcq.load(Tb.rcq, Tb.ncls);
+    mdq.load(Tb.mdq, nmd);
     tab.load(src, n16);
     inv.store(s_inv, Tb.inv, Tb.P);
     pool.store(s_pool, Tb.pool, Tb.npool);
     cls.store(s_cls, Tb.cls, Tb.P);
     rcq.store(s_rcq, Tb.rcq, Tb.ncls);
+    mdq.store(s_md, Tb.mdq, nmd);
     tab.store(reinterpret_cast<u32x4 *>(s_tab), src, n16);
   }
 };

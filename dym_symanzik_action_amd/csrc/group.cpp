@@ -118,6 +118,7 @@ void build_classes(Group &g) {
   g.cprod.clear();
   g.rc.clear();
   g.rcq.clear();
+  g.mdq.clear();
   g.cval.clear();
   g.cval0.clear();
   g.dmax = g.rmax = 0;
@@ -155,6 +156,24 @@ void build_classes(Group &g) {
   g.cprod.resize(g.pool.size() * P);
   for (size_t j = 0; j < g.pool.size(); ++j)
     for (size_t t = 0; t < P; ++t) g.cprod[j * P + t] = g.cls[g.mult[(size_t)g.pool[j] * P + t]];
+  // per-class bound on one hit's change of a staple term: a trial term is
+  // ReTr(b p r) = ReTr(p' (b r)) with p' = b p b^-1, which is in the pool
+  // when the pool is closed under conjugation
+  std::vector<char> in_pool(P, 0);
+  for (uint16_t p : g.pool) in_pool[p] = 1;
+  bool closed = true;
+  for (size_t j = 0; j < g.pool.size() && closed; ++j)
+    for (size_t y = 0; y < P && closed; ++y)
+      closed = in_pool[g.mult[g.mult[y * P + g.pool[j]] * P + g.inv[y]]] != 0;
+  if (closed) {
+    std::vector<double> md(nc, 0.0);
+    for (size_t t = 0; t < P; ++t)
+      for (uint16_t p : g.pool) {
+        const double d = std::fabs(g.retr[g.mult[(size_t)p * P + t]] - g.retr[t]);
+        md[cls[t]] = std::max(md[cls[t]], d);
+      }
+    for (double v : md) g.mdq.push_back((int32_t)std::ceil(std::ldexp(v, 24)) + 1);
+  }
   if (g.rmax < 3.999) {
     std::vector<int16_t> v16;
     for (double v : g.rc) v16.push_back((int16_t)std::lround(std::ldexp(v, 13)));

@@ -34,6 +34,12 @@ struct Group {
   std::vector<uint8_t> cls, cprod;
   std::vector<double> rc;
   std::vector<int32_t> rcq;
+  // mdq[c]: max over elements t of class c and pool elements p of
+  // |ReTr(p t) - ReTr(t)|, in units of 2^-24 rounded up: bounds how much one
+  // hit can move a staple term whose staple-times-link product is in class c
+  // (empty unless the pool is closed under conjugation, which makes the bound
+  // hold for the reference's element values)
+  std::vector<int32_t> mdq;
   std::vector<int16_t> cval, cval0;
   double dmax = 0, rmax = 0;
 };

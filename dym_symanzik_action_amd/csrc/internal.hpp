@@ -20,7 +20,7 @@ struct DeviceTables {
   uint16_t *mult = nullptr, *inv = nullptr, *pool = nullptr;
   double *retr = nullptr, *imtr = nullptr;
   uint8_t *cls = nullptr, *cprod = nullptr;  // class tables (null if the group has none)
-  int32_t *rcq = nullptr;
+  int32_t *rcq = nullptr, *mdq = nullptr;
   int16_t *cval = nullptr, *cval0 = nullptr;
   float *mat = nullptr;  // FP32 SU(3) matrices, 20 floats per element (null if none)
 };

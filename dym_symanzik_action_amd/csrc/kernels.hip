@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs A) {
 // moves still gather from the global table.
 constexpr int kClsBlockNarrow = 512, kClsBlockWide = 1024;
 
-template <int D, bool W>
+template <int D, bool W, bool B>
 __global__ __launch_bounds__(W ? kClsBlockWide : kClsBlockNarrow) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_sweep_cls(SweepArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -61,6 +61,8 @@ void k_sweep_cls(SweepArgs A) {
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(reinterpret_cast<unsigned char *>(s_inv) + pad16(2 * (size_t)P));
   unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
   unsigned char *s_tab = s_tab0 + pad16((W ? 2 : 1) * (size_t)P);
+  // narrow form: the per-class hit bounds after the class table (null if none)
+  int32_t *s_md = !W && A.T.mdq ? reinterpret_cast<int32_t *>(s_tab + pad16((size_t)np * P)) : nullptr;
   if constexpr (W) {
     // 1024 lanes: every table in at most 10 rounds
     Staged<2, uint16_t> g_inv;
@@ -79,7 +81,7 @@ void k_sweep_cls(SweepArgs A) {
     g_tab.store(reinterpret_cast<u32x4 *>(s_tab), c1, n1);
   } else {
     ClsNarrowStage g;
-    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
+    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
   }
   __syncthreads();
 
@@ -92,13 +94,13 @@ void k_sweep_cls(SweepArgs A) {
   if constexpr (W) {
     Lk = ClsWide{reinterpret_cast<const int16_t *>(s_tab), reinterpret_cast<const int16_t *>(s_tab0)};
   } else {
-    Lk = ClsNarrow{s_tab, s_tab0, s_rcq};
+    Lk = ClsNarrow{s_tab, s_tab0, s_rcq, s_md};
   }
   unsigned long long acc = 0;
   uint32_t nfall = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride)
-    acc += update_link_cls<D>(C, Lk, K, A.G, T, A.mu, A.sites[k], nfall);
+    acc += update_link_cls<D, B>(C, Lk, K, A.G, T, A.mu, A.sites[k], nfall);
   acc = wave_sum_u64(acc);
   const unsigned long long nf = wave_sum_u64(nfall);
   if ((threadIdx.x & 63u) == 0) {
@@ -163,7 +165,7 @@ constexpr size_t cls_small_static_lds(int nl) {
   return 64 * sizeof(int32_t) + (size_t)kNHit * nl * (sizeof(double) + sizeof(uint16_t));
 }
 
-template <int D, int NL>
+template <int D, int NL, bool B>
 __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t s_rcq[64];
@@ -174,9 +176,10 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
   uint16_t *s_pool = reinterpret_cast<uint16_t *>(smem + pad16(2 * (size_t)P));
   unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
   unsigned char *s_tab = s_tab0 + pad16(P);
+  int32_t *s_md = A.T.mdq ? reinterpret_cast<int32_t *>(s_tab + pad16((size_t)np * P)) : nullptr;
   {
     ClsNarrowStage g;
-    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
+    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
   }
   const uint32_t first = blockIdx.x * NL;
   const uint32_t n = A.count - first < (uint32_t)NL ? A.count - first : (uint32_t)NL;
@@ -196,12 +199,12 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
   const LinkCtx C{A.links, A.mstate, nullptr, s_pool, npd, np, A.beta0, A.beta1, A.beta2,
                   A.seed, A.sweep_lo, A.sweep_hi, A.rng};
   const ClsCtx K{A.T.retr, A.cmargin};
-  const ClsNarrow Lk{s_tab, s_tab0, s_rcq};
+  const ClsNarrow Lk{s_tab, s_tab0, s_rcq, s_md};
   unsigned long long acc = 0;
   uint32_t nfall = 0;
   if (threadIdx.x < n) {
     const PreDraws pre{s_jp, s_u2, (uint32_t)NL, threadIdx.x};
-    acc = update_link_cls<D>(C, Lk, K, A.G, T, A.mu, A.sites[first + threadIdx.x], nfall, &pre);
+    acc = update_link_cls<D, B>(C, Lk, K, A.G, T, A.mu, A.sites[first + threadIdx.x], nfall, &pre);
   }
   acc = wave_sum_u64(acc);
   const unsigned long long nf = wave_sum_u64(nfall);
@@ -231,19 +234,25 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     uint16_t *s_pool = reinterpret_cast<uint16_t *>(smem + pad16(2 * (size_t)P));
     unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
     unsigned char *s_tab = s_tab0 + pad16(P);
+    int32_t *s_md = BC.T.mdq ? reinterpret_cast<int32_t *>(s_tab + pad16((size_t)np * P)) : nullptr;
     {
       ClsNarrowStage g;
-      g.run(BC.T, s_inv, s_pool, s_tab0, s_rcq, s_tab);
+      g.run(BC.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
     }
     __syncthreads();
     const TabView T{BC.T.mult, s_inv, P};
     const LinkCtx C{BC.links, BC.mstate, nullptr, s_pool, (double)np, np, BC.beta0, BC.beta1, BC.beta2,
                     BC.seed, (uint32_t)sweep, (uint32_t)(sweep >> 32), BC.rng};
     const ClsCtx K{BC.T.retr, BC.cmargin};
-    const ClsNarrow Lk{s_tab, s_tab0, s_rcq};
+    const ClsNarrow Lk{s_tab, s_tab0, s_rcq, s_md};
     uint32_t nfall = 0;
-    for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
-      acc += update_link_cls<D>(C, Lk, K, BC.G, T, tk.mu, BC.sites[tk.offset + t], nfall);
+    if (BC.bound && s_md) {  // uniform over the block
+      for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
+        acc += update_link_cls<D, true>(C, Lk, K, BC.G, T, tk.mu, BC.sites[tk.offset + t], nfall);
+    } else {
+      for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
+        acc += update_link_cls<D, false>(C, Lk, K, BC.G, T, tk.mu, BC.sites[tk.offset + t], nfall);
+    }
     const unsigned long long nf = wave_sum_u64(nfall);
     if ((threadIdx.x & 63u) == 0 && nf) atomicAdd(BC.fall, nf);
   } else {
@@ -538,7 +547,7 @@ size_t sweep_cls_lds_bytes(const Tables &T, int mode) {
     return pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool) + pad16(2 * (size_t)T.P) +
            pad16(2 * (size_t)T.npool * T.P);
   return pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool) + pad16(T.P) +
-         pad16((size_t)T.npool * T.P);
+         pad16((size_t)T.npool * T.P) + (T.mdq ? pad16(4 * (size_t)T.ncls) : 0);
 }
 
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
@@ -551,12 +560,17 @@ hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
     if (!A.fall || !A.T.cprod) return hipErrorInvalidValue;
     const uint32_t nl = sm ? 128 : 256, nb = (A.count + nl - 1) / nl;
     const size_t lds = lds_small;
+    const bool bnd = A.bound && A.T.mdq;
 #define DSA_SMALL(d)                                                                                       \
   case d:                                                                                                \
-    if (sm)                                                                                              \
-      hipLaunchKernelGGL((k_sweep_cls_small<d, 128>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);       \
+    if (sm && bnd)                                                                                       \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 128, true>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
+    else if (sm)                                                                                         \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 128, false>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
+    else if (bnd)                                                                                        \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 256, true>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_sweep_cls_small<d, 256>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);       \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 256, false>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
     break;
     switch (A.G.D) {
       DSA_SMALL(2)
@@ -589,10 +603,12 @@ hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
     uint32_t nb = (A.count + bs - 1) / bs;
     if (max_blocks > 0 && nb > (uint32_t)max_blocks) nb = (uint32_t)max_blocks;
     const size_t lds = sweep_cls_lds_bytes(A.T, A.use_cls);
+    const bool bnd = A.bound && A.T.mdq;
 #define DSA_CLS_CASE(DD)                                                                        \
   case DD:                                                                                      \
-    if (w) hipLaunchKernelGGL((k_sweep_cls<DD, true>), dim3(nb), dim3(bs), lds, s, A);           \
-    else hipLaunchKernelGGL((k_sweep_cls<DD, false>), dim3(nb), dim3(bs), lds, s, A);            \
+    if (w) hipLaunchKernelGGL((k_sweep_cls<DD, true, false>), dim3(nb), dim3(bs), lds, s, A);    \
+    else if (bnd) hipLaunchKernelGGL((k_sweep_cls<DD, false, true>), dim3(nb), dim3(bs), lds, s, A); \
+    else hipLaunchKernelGGL((k_sweep_cls<DD, false, false>), dim3(nb), dim3(bs), lds, s, A);     \
     break;
     switch (A.G.D) {
       DSA_CLS_CASE(2)

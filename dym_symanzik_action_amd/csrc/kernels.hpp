@@ -31,6 +31,7 @@ struct Tables {
   // ReTr per class in units of 2^-24
   const uint8_t *cls, *cprod;
   const int32_t *rcq;
+  const int32_t *mdq;  // per class, the bound of Group::mdq (null if none)
   uint32_t ncls;
   // wide form: class values in units of 2^-13 (null if absent)
   const int16_t *cval, *cval0;
@@ -62,6 +63,12 @@ struct SweepArgs {
   int use_cls;
   double cmargin, cmargin_acc;
   unsigned long long *fall;
+  // narrow class form: test each hit on its plaquette terms first, with the
+  // rectangle terms bounded (rejects without the rectangle lookups; pays
+  // where nearly every hit of a wavefront is rejected that way, i.e. at very
+  // low acceptance -- a wavefront runs the rectangle terms if any lane needs
+  // them); identical results either way
+  int bound;
   // the device's LDS bytes per block: the small-pass class forms run only
   // where their draw arrays fit beside the tables
   size_t lds_max;
@@ -89,7 +96,12 @@ struct BatchChain {
   int cls;
   double cmargin;
   unsigned long long *fall;
+  int bound;  // as SweepArgs::bound
 };
+
+// The plaquette-first test is chosen for a sweep call when the chain's
+// acceptance over its previous call was below this.
+constexpr double kBoundAcceptance = 1e-3;
 
 // A block's work: `count` (<= 256) consecutive sites of one chain's pass.
 struct BatchTask {

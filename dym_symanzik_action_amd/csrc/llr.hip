@@ -570,20 +570,25 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
           const uint32_t cn = n - base < 64 ? n - base : 64;
           const uint32_t kk = base + (uint32_t)lane;
           const bool valid = (uint32_t)lane < cn;
-          const long long q1 = valid ? sP1[kk] : 0, q2 = valid ? sP2[kk] : 0;
+          // the chunk's prefix sums stay in registers; the walk reads them
+          // across lanes (readlane) instead of from LDS
+          const long long c1 = base ? sP1[base - 1] : 0, c2 = base ? sP2[base - 1] : 0;  // before the chunk
+          const int cN = base ? sPN[base - 1] : 0;
+          const long long q1 = valid ? sP1[kk] : c1, q2 = valid ? sP2[kk] : c2;
+          const int qn = valid ? sPN[kk] : cN;
           const long long e1 = valid ? q1 - (kk ? sP1[kk - 1] : 0) : 0, e2 = valid ? q2 - (kk ? sP2[kk - 1] : 0) : 0;
           const bool chg = valid && sChg[kk];
           uint32_t s0 = base < f0 ? f0 - base : 0;  // uniform over the wave
           while (s0 < cn) {
-            const uint32_t at = base + s0;
-            const long long B1 = at ? sP1[at - 1] : 0, B2 = at ? sP2[at - 1] : 0;
+            const long long B1 = s0 ? readlane64(q1, s0 - 1) : c1, B2 = s0 ? readlane64(q2, s0 - 1) : c2;
             const long long N1 = E1 + q1 - B1, N2 = E2 + q2 - B2;
             const unsigned long long bm = __ballot(chg && (uint32_t)lane >= s0 && !R(N1 - e1, N2 - e2, N1, N2));
             const uint32_t f = bm ? (uint32_t)__builtin_ctzll(bm) : cn;
             if (f > s0) {  // accept [s0, f)
-              E1 += sP1[base + f - 1] - B1;
-              E2 += sP2[base + f - 1] - B2;
-              acc += (unsigned long long)(sPN[base + f - 1] - (at ? sPN[at - 1] : 0));
+              E1 += readlane64(q1, f - 1) - B1;
+              E2 += readlane64(q2, f - 1) - B2;
+              acc += (unsigned long long)(__builtin_amdgcn_readlane(qn, (int)(f - 1)) -
+                                          (s0 ? __builtin_amdgcn_readlane(qn, (int)(s0 - 1)) : cN));
             }
             if (f >= cn) break;
             const unsigned long long gm = __ballot(chg && (uint32_t)lane > f && R(E1, E2, E1 + e1, E2 + e2));

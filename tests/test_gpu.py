@@ -116,6 +116,10 @@ def test_cli_replay_reproduces_reference_output():
     ("S108", 3, 6, 4, 0.0, 1.5, 0.2, 12, "hot", 4),
     ("myBT", 2, 8, 6, 0.0, 2.0, 0.1, 13, "hot", 5),
     ("mys1080-v4", 4, 5, 4, 0.0, 2.0, -0.1, 14, "hot", 2),
+    # frozen (acceptance ~0): from the second sweep call on, the narrow class
+    # form runs its plaquette-first test
+    ("mys1080-v4", 4, 6, 4, 0.0, 3.5, 0.0, 15, "cold", 4),
+    ("mys1080-v4", 4, 4, 6, 0.0, 4.0, -0.3, 16, "cold", 3),
 ])
 @pytest.mark.parametrize("rng", ["philox", "minstd"])
 @pytest.mark.parametrize("kernel", ["table", "class", "class-narrow", "class-all-exact"])
