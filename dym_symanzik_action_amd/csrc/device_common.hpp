@@ -14,6 +14,17 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// The pointer as one into global memory, so that accesses through it are
+// global (not flat) instructions.  Pointers read from device-side structs
+// (the batch and LLR chains) are generic to the compiler, and a flat access
+// counts against the LDS wait counter too: every wait for an LDS lookup
+// would also wait for all the global loads in flight.  Only for pointers
+// into global memory (flat and global addresses coincide there).
+template <class T>
+__device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD one
 // contiguous range of the pass so neighbouring links share an L2.
 __device__ inline uint32_t xcd_block() {
@@ -95,14 +106,14 @@ __device__ inline uint32_t U(const uint16_t *links, const Geom &G, const Site<D>
                              int a, int nu, int b) {
   uint32_t par;
   const uint32_t site = shifted<D>(G, s, mu, a, nu, b, &par);
-  return links[(size_t)dir * G.V + lpos(G, site, par)];
+  return gmem(links)[(size_t)dir * G.V + lpos(G, site, par)];
 }
 
 struct TabView {
   const uint16_t *__restrict__ mult;
   const uint16_t *inv;  // LDS or global
   uint32_t P;
-  __device__ inline uint32_t M(uint32_t a, uint32_t b) const { return mult[a * P + b]; }
+  __device__ inline uint32_t M(uint32_t a, uint32_t b) const { return gmem(mult)[a * P + b]; }
   __device__ inline uint32_t I(uint32_t a) const { return inv[a]; }
 };
 
@@ -254,7 +265,7 @@ __device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const Ta
   uint32_t st[NS], re[NR];
   build_staples<D>(C.links, G, T, s, mu, st, re);
   const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
-  uint32_t b = C.links[lp];
+  uint32_t b = gmem(C.links)[lp];
   const uint32_t b0 = b;
   double r1 = 0, m1 = 0;
 #pragma unroll
@@ -262,7 +273,7 @@ __device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const Ta
 #pragma unroll
   for (int q = 0; q < NR; ++q) m1 += C.s_retr[T.M(b, re[q])];
   uint32_t ms = 0;
-  if (C.rng == kRngMinstd) ms = C.mstate[i];
+  if (C.rng == kRngMinstd) ms = gmem(C.mstate)[i];
   const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
   uint32_t acc = 0;
   U4 w{0, 0, 0, 0};
@@ -293,8 +304,8 @@ __device__ inline uint32_t update_link(const LinkCtx &C, const Geom &G, const Ta
       ++acc;
     }
   }
-  if (b != b0) C.links[lp] = (uint16_t)b;
-  if (C.rng == kRngMinstd) C.mstate[i] = ms;
+  if (b != b0) gmem(C.links)[lp] = (uint16_t)b;
+  if (C.rng == kRngMinstd) gmem(C.mstate)[i] = ms;
   return acc;
 }
 
@@ -345,7 +356,7 @@ __device__ inline int exact_hit(const LinkCtx &C, const double *retr, const TabV
 #pragma unroll
   for (int k = 0; k < NL; ++k) {
     const uint32_t tk = t[k];
-    const double v = retr[T.M(T.M(b, tk), bi)], vn = retr[T.M(T.M(bn, tk), bi)];
+    const double v = gmem(retr)[T.M(T.M(b, tk), bi)], vn = gmem(retr)[T.M(T.M(bn, tk), bi)];
     if (k < NS) {
       e1 += v;
       e1n += vn;
@@ -492,10 +503,10 @@ __device__ inline void cls_prepare(ClsLink<D> &S, const LinkCtx &C, const L &Lk,
   S.i = i;
   S.s = decode<D>(G, i);
   S.lp = (size_t)mu * G.V + lpos(G, i, S.s.par);
-  S.b = S.b0 = C.links[S.lp];
+  S.b = S.b0 = gmem(C.links)[S.lp];
   build_t<D>(C.links, G, T, S.s, mu, S.b, S.t);
   cls_sums_bound<D, BOUND>(Lk, S);
-  S.ms = C.rng == kRngMinstd ? C.mstate[i] : 0u;
+  S.ms = C.rng == kRngMinstd ? gmem(C.mstate)[i] : 0u;
 }
 
 // BOUND (the narrow form only): each hit is first tested on its plaquette
@@ -564,8 +575,8 @@ __device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk
       ++acc;
     }
   }
-  if (S.b != S.b0) C.links[S.lp] = (uint16_t)S.b;
-  if (C.rng == kRngMinstd) C.mstate[S.i] = S.ms;
+  if (S.b != S.b0) gmem(C.links)[S.lp] = (uint16_t)S.b;
+  if (C.rng == kRngMinstd) gmem(C.mstate)[S.i] = S.ms;
   return acc;
 }
 
@@ -683,10 +694,10 @@ __device__ inline uint32_t update_link_mat(const LinkCtx &C, const MatCtx &K, co
   constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
   const Site<D> s = decode<D>(G, i);
   const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
-  uint32_t b = C.links[lp];
+  uint32_t b = gmem(C.links)[lp];
   const uint32_t b0 = b;
   uint32_t ms = 0;
-  if (C.rng == kRngMinstd) ms = C.mstate[i];
+  if (C.rng == kRngMinstd) ms = gmem(C.mstate)[i];
   const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
   // hit h's proposal index and acceptance uniform, in stream order
   U4 w{0, 0, 0, 0};
@@ -753,8 +764,8 @@ __device__ inline uint32_t update_link_mat(const LinkCtx &C, const MatCtx &K, co
       }
     }
   }
-  if (b != b0) C.links[lp] = (uint16_t)b;
-  if (C.rng == kRngMinstd) C.mstate[i] = ms;
+  if (b != b0) gmem(C.links)[lp] = (uint16_t)b;
+  if (C.rng == kRngMinstd) gmem(C.mstate)[i] = ms;
   return acc;
 }
 
@@ -776,7 +787,7 @@ struct Staged {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       const uint32_t t = threadIdx.x + (uint32_t)j * blockDim.x;
-      if (t < n) r[j] = src[t];
+      if (t < n) r[j] = gmem(src)[t];
     }
   }
   __device__ inline void store(T *dst, const T *__restrict__ src, uint32_t n) const {
@@ -785,7 +796,7 @@ struct Staged {
       const uint32_t t = threadIdx.x + (uint32_t)j * blockDim.x;
       if (t < n) dst[t] = r[j];
     }
-    for (uint32_t t = threadIdx.x + (uint32_t)K * blockDim.x; t < n; t += blockDim.x) dst[t] = src[t];
+    for (uint32_t t = threadIdx.x + (uint32_t)K * blockDim.x; t < n; t += blockDim.x) dst[t] = gmem(src)[t];
   }
 };
 
@@ -848,7 +859,7 @@ __device__ inline uint32_t site_parity(const Geom &G, uint32_t i) {
 // ---------------------------------------------------------------- loops
 
 __device__ inline uint32_t link_at(const uint16_t *links, const Geom &G, uint32_t site, int dir) {
-  return links[(size_t)dir * G.V + lpos(G, site, site_parity(G, site))];
+  return gmem(links)[(size_t)dir * G.V + lpos(G, site, site_parity(G, site))];
 }
 
 __device__ inline uint32_t step_site(const Geom &G, uint32_t i, int d, int sgn) {

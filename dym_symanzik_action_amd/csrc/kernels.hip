@@ -248,10 +248,10 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     uint32_t nfall = 0;
     if (BC.bound && s_md) {  // uniform over the block
       for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
-        acc += update_link_cls<D, true>(C, Lk, K, BC.G, T, tk.mu, BC.sites[tk.offset + t], nfall);
+        acc += update_link_cls<D, true>(C, Lk, K, BC.G, T, tk.mu, gmem(BC.sites)[tk.offset + t], nfall);
     } else {
       for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
-        acc += update_link_cls<D, false>(C, Lk, K, BC.G, T, tk.mu, BC.sites[tk.offset + t], nfall);
+        acc += update_link_cls<D, false>(C, Lk, K, BC.G, T, tk.mu, gmem(BC.sites)[tk.offset + t], nfall);
     }
     const unsigned long long nf = wave_sum_u64(nfall);
     if ((threadIdx.x & 63u) == 0 && nf) atomicAdd(BC.fall, nf);
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     const LinkCtx C{BC.links, BC.mstate, s_retr, s_pool, (double)np, np, BC.beta0, BC.beta1, BC.beta2,
                     BC.seed, (uint32_t)sweep, (uint32_t)(sweep >> 32), BC.rng};
     for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
-      acc += update_link<D>(C, BC.G, T, tk.mu, BC.sites[tk.offset + t]);
+      acc += update_link<D>(C, BC.G, T, tk.mu, gmem(BC.sites)[tk.offset + t]);
   }
   acc = wave_sum_u64(acc);
   if ((threadIdx.x & 63u) == 0 && acc) atomicAdd(&acc_out[tk.chain], acc);

@@ -29,6 +29,14 @@ __device__ inline long long wave_sum_i64(long long v) {
 }
 
 
+// a value that is the same in every lane, made known to the compiler as
+// such (it then lives in scalar registers and its branches are scalar)
+__device__ inline long long uniform64(long long v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 // lane k's value (k uniform over the wave)
 __device__ inline long long readlane64(long long v, uint32_t k) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)k);
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
     const double npd = (double)np;
     for (uint32_t j = threadIdx.x; j < n * (uint32_t)kNHit; j += blockDim.x) {
       const uint32_t l = j % n, h = j / n;
-      const uint32_t lk = BC.sites[tk.offset + l] * (uint32_t)D + (uint32_t)tk.mu;
+      const uint32_t lk = gmem(BC.sites)[tk.offset + l] * (uint32_t)D + (uint32_t)tk.mu;
       const U4 w = philox4x32_10(U4{lk, h, sl, sh}, BC.seed, kKeyLLR);
       s_jp[h][l] = (uint16_t)(npd * u53(w.x, w.y));
       s_u2[h][l] = u53(w.z, w.w);
@@ -216,11 +224,11 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
   const double npool_d = (double)np;
   for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x) {
     const uint32_t idx = tk.offset + t;
-    const uint32_t i = BC.sites[idx];
+    const uint32_t i = gmem(BC.sites)[idx];
     const int mu = tk.mu;
     const Site<D> s = decode<D>(G, i);
     const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
-    uint32_t b = BC.links[lp];
+    uint32_t b = gmem(BC.links)[lp];
     const uint32_t b0 = b;
     const uint32_t link = i * (uint32_t)D + (uint32_t)mu;
     uint32_t nacc = 0;
@@ -286,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
         }
       }
       if (b != b0) {  // energy change in fixed point: s_k = t_k b^-1
-        BC.links[lp] = (uint16_t)b;
+        gmem(BC.links)[lp] = (uint16_t)b;
         const uint32_t bi = T.I(b);
         long long p1 = 0, p2 = 0, q1 = 0, q2 = 0;
 #pragma unroll
@@ -354,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
         }
       }
       if (b != b0) {
-        BC.links[lp] = (uint16_t)b;
+        gmem(BC.links)[lp] = (uint16_t)b;
         long long p1 = 0, p2 = 0;
 #pragma unroll
         for (int j = 0; j < NS; ++j) p1 += to_fix(s_retr[T.M(b, stp[j])]);
@@ -364,10 +372,10 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
         d2 = p2 - q2;
       }
     }
-    LC.old[idx] = (uint16_t)b0;
-    LC.dE1[idx] = d1;
-    LC.dE2[idx] = d2;
-    LC.nacc[idx] = (uint8_t)(nacc | (b != b0 ? (uint32_t)kChanged : 0u));
+    gmem(LC.old)[idx] = (uint16_t)b0;
+    gmem(LC.dE1)[idx] = d1;
+    gmem(LC.dE2)[idx] = d2;
+    gmem(LC.nacc)[idx] = (uint8_t)(nacc | (b != b0 ? (uint32_t)kChanged : 0u));
   }
 }
 
@@ -410,6 +418,24 @@ __device__ inline long long scan_i64(long long v, int lane) {
   return v + (row == 0 ? 0 : row == 1 ? t0 : row == 2 ? t0 + t1 : t0 + t1 + t2);
 }
 
+// Inclusive prefix sums within each row of 16 lanes (lanes shifted in from
+// outside a row read 0).
+__device__ inline long long row_scan_i64(long long v) {
+  v += dpp_shift64<0x111>(v);
+  v += dpp_shift64<0x112>(v);
+  v += dpp_shift64<0x114>(v);
+  v += dpp_shift64<0x118>(v);
+  return v;
+}
+
+__device__ inline int row_scan_i32(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  return v;
+}
+
 __device__ inline int scan_i32(int v, int lane) {
   v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
@@ -429,11 +455,104 @@ __device__ inline uint32_t block_first(bool pred, uint32_t n, unsigned *slots, u
   const unsigned long long bm = __ballot(pred);
   if ((threadIdx.x & 63u) == 0 && bm) atomicMin(&slots[k], (threadIdx.x & ~63u) + (unsigned)__builtin_ctzll(bm));
   __syncthreads();
-  const unsigned r = slots[k];
+  const unsigned r = (unsigned)__builtin_amdgcn_readfirstlane((int)slots[k]);
   const uint32_t nx = k == 2 ? 0 : k + 1;
   if (threadIdx.x == 0) slots[nx == 2 ? 0 : nx + 1] = 0xffffffffu;
   k = nx;
   return r < n ? r : n;
+}
+
+// The contested walk under the window rule (every pass that is not a
+// drive-in pass): a changed proposal k is accepted iff the energy after it is
+// inside the window (axes 1, 2: the ruling energy; axis 3: both, the box),
+// i.e. (unsigned) (E + e_k - lo) <= hi - lo.  With q the inclusive prefix
+// sums of the changes, a run of acceptances from lane s, met with energy E_s,
+// has energy lo + o + q_k after lane k (o = E_s - lo - q_{s-1}), so it ends at
+// the first changed lane whose o + q_k leaves [0, hi - lo]: one add and one
+// compare per lane against uniform values.  A run of refusals from a failing
+// lane f leaves E unchanged and ends at the first changed lane after f whose
+// own change keeps E inside.  The walk follows the ruling energies; the other
+// energy and the hit count gain q_{f-1} - q_{s-1} per acceptance run (read
+// across lanes beside the walk's own reads); the lanes of the acceptance runs
+// are collected in a mask and the refused ones restore their links.
+template <int AX>
+__device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const long long *sP1, const long long *sP2,
+                            const int *sPN, const uint8_t *sChg, const uint32_t *sPos, const uint16_t *sOld,
+                            uint16_t *links, long long &E1, long long &E2, unsigned long long &acc) {
+  const int lane = (int)(threadIdx.x & 63u);
+  n = (uint32_t)__builtin_amdgcn_readfirstlane((int)n);
+  f0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)f0);
+  E1 = uniform64(E1);
+  E2 = uniform64(E2);
+  const unsigned long long Wa = (unsigned long long)(R.hi - R.lo), Wb = (unsigned long long)(R.hi2 - R.lo2);
+  for (uint32_t base = f0 & ~63u; base < n; base += 64) {  // n <= kDecide: base + 63 < kDecide
+    const uint32_t cn = n - base < 64 ? n - base : 64;
+    const uint32_t kk = base + (uint32_t)lane, kp = kk ? kk - 1 : 0;
+    const bool valid = (uint32_t)lane < cn;
+    // every load in bounds and unconditional, so that they wait together
+    const long long P1 = sP1[kk], P2 = sP2[kk], P1p = sP1[kp], P2p = sP2[kp];
+    const int qn = sPN[kk];
+    const uint8_t ch = sChg[kk];
+    const uint32_t kb = base ? base - 1 : 0;
+    const long long B1 = sP1[kb], B2 = sP2[kb];
+    const int BN = sPN[kb];
+    // before the chunk
+    const long long c1 = base ? uniform64(B1) : 0, c2 = base ? uniform64(B2) : 0;
+    const int cN = base ? __builtin_amdgcn_readfirstlane(BN) : 0;
+    const long long q1 = P1, q2 = P2;
+    const long long e1 = valid ? P1 - (kk ? P1p : 0) : 0, e2 = valid ? P2 - (kk ? P2p : 0) : 0;
+    const unsigned long long cm = __ballot(valid & (ch != 0));  // changed lanes
+    const uint32_t s_init = base < f0 ? f0 - base : 0;          // lanes before it are settled
+    // the ruling energy: S1 on axes 1 and 3, S2 on axis 2; the box adds S2;
+    // the other one (axes 1, 2) is summed by runs
+    const long long qa = AX == 2 ? q2 : q1, ea = AX == 2 ? e2 : e1, ca = AX == 2 ? c2 : c1;
+    const long long qo = AX == 2 ? q1 : q2, co = AX == 2 ? c1 : c2;
+    long long so = 0;  // gain of the other energy over the chunk
+    int sn = 0;        // hits of the accepted changed proposals
+    uint32_t s = s_init;  // uniform over the wave, < 64
+    long long o = (AX == 2 ? E2 : E1) - R.lo - (s ? readlane64(qa, s - 1) : ca);
+    long long ob = AX == 3 ? E2 - R.lo2 - (s ? readlane64(q2, s - 1) : c2) : 0;
+    long long xa, xb = 0;  // the energies minus lo at the chunk's end
+    unsigned long long amask = 0;
+    for (;;) {
+      bool out = (unsigned long long)(qa + o) > Wa;
+      if (AX == 3) out = out | ((unsigned long long)(q2 + ob) > Wb);
+      const unsigned long long from_s = ~0ull << s;
+      const unsigned long long fm = __ballot(out) & cm & from_s;
+      const long long os = s ? readlane64(qo, s - 1) : co;  // before the run
+      const int ns = s ? __builtin_amdgcn_readlane(qn, (int)(s - 1)) : cN;
+      if (!fm) {  // the rest of the chunk is accepted
+        amask |= from_s;
+        xa = o + readlane64(qa, cn - 1);
+        if (AX == 3) xb = ob + readlane64(q2, cn - 1);
+        if (AX != 3) so += readlane64(qo, cn - 1) - os;
+        sn += __builtin_amdgcn_readlane(qn, (int)(cn - 1)) - ns;
+        break;
+      }
+      const uint32_t f = (uint32_t)__builtin_ctzll(fm);
+      amask |= from_s & ((1ull << f) - 1);
+      xa = o + (f ? readlane64(qa, f - 1) : ca);  // the energy met by lane f
+      if (AX == 3) xb = ob + (f ? readlane64(q2, f - 1) : c2);
+      if (AX != 3) so += (f ? readlane64(qo, f - 1) : co) - os;
+      sn += (f ? __builtin_amdgcn_readlane(qn, (int)(f - 1)) : cN) - ns;
+      bool in = (unsigned long long)(ea + xa) <= Wa;
+      if (AX == 3) in = in & ((unsigned long long)(e2 + xb) <= Wb);
+      const unsigned long long gm = __ballot(in) & cm & (f < 63 ? ~0ull << (f + 1) : 0ull);
+      if (!gm) break;  // the rest of the chunk is refused: the energy stays
+      const uint32_t g = (uint32_t)__builtin_ctzll(gm);
+      o = xa - readlane64(qa, g - 1);
+      if (AX == 3) ob = xb - readlane64(q2, g - 1);
+      s = g;
+    }
+    const bool took = ((amask >> lane) & 1ull) != 0;
+    if ((uint32_t)lane >= s_init && ((cm >> lane) & 1ull) && !took) gmem(links)[sPos[kk]] = sOld[kk];
+    if (AX == 1) E2 += so;
+    if (AX == 2) E1 += so;
+    if (AX != 2) E1 = xa + R.lo;
+    if (AX == 2) E2 = xa + R.lo;
+    if (AX == 3) E2 = xb + R.lo2;
+    acc += (unsigned long long)(unsigned)sn;
+  }
 }
 
 __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
@@ -472,11 +591,11 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
     In v{0, 0, 0, 0, 0};
     if (seg < pr.count && tid < pr.count - seg) {
       const uint32_t idx = pr.offset + seg + tid;
-      v.d1 = LC.dE1[idx];
-      v.d2 = LC.dE2[idx];
-      v.raw = (int)LC.nacc[idx];
-      v.pos = LC.pos[idx];
-      v.old = LC.old[idx];
+      v.d1 = gmem(LC.dE1)[idx];
+      v.d2 = gmem(LC.dE2)[idx];
+      v.raw = (int)gmem(LC.nacc)[idx];
+      v.pos = gmem(LC.pos)[idx];
+      v.old = gmem(LC.old)[idx];
     }
     return v;
   };
@@ -521,7 +640,7 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
       __syncthreads();  // the wave slots are written again below
       const long long Eend = E + (R.axis == 1 ? a1 : a2);
       if (sgn != 0 && (sgn > 0 ? Eend <= R.mid : Eend >= R.mid)) {
-        if (changed && !toward) links[cur.pos] = cur.old;
+        if (changed && !toward) gmem(links)[cur.pos] = cur.old;
         E1 += a1;
         E2 += a2;
         acc += (unsigned long long)nt;
@@ -541,15 +660,20 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
     sPos[tid] = cur.pos;
     sChg[tid] = changed ? 1 : 0;
     __syncthreads();
-    for (int w = 0; w < wv; ++w) {
-      p1 += sW1[w];
-      p2 += sW2[w];
-      pn += sWN[w];
-    }
-    if (!R.drive) {  // unchanged links: all accepted under the window rule
-      int tot = 0;
-      for (int w = 0; w < kDecideWaves; ++w) tot += sWT[w];
-      acc += (unsigned long long)tot;
+    {
+      // the waves' totals, one per lane of each row of 16, scanned along the
+      // row: lane w - 1 holds the sum over the waves below wave w
+      static_assert(kDecideWaves == 16, "one row of 16 lanes per wave's totals");
+      const int wl = lane & 15;
+      const long long t1 = row_scan_i64(sW1[wl]), t2 = row_scan_i64(sW2[wl]);
+      const int tn = row_scan_i32(sWN[wl]), tt = row_scan_i32(sWT[wl]);
+      const uint32_t wu = (uint32_t)__builtin_amdgcn_readfirstlane(wv);
+      if (wu > 0) {
+        p1 += readlane64(t1, wu - 1);
+        p2 += readlane64(t2, wu - 1);
+        pn += __builtin_amdgcn_readlane(tn, (int)(wu - 1));
+      }
+      if (!R.drive) acc += (unsigned long long)(unsigned)__builtin_amdgcn_readlane(tt, 15);  // unchanged links: all accepted under the window rule
     }
     sP1[tid] = p1;
     sP2[tid] = p2;
@@ -558,14 +682,26 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
     // the whole segment at once: the first proposal whose rule fails
     const uint32_t f0 = block_first(changed && !R(E1 + p1 - d1, E2 + p2 - d2, E1 + p1, E2 + p2), n, slots, k);
     if (f0 > 0) {
-      E1 += sP1[f0 - 1];
-      E2 += sP2[f0 - 1];
-      acc += (unsigned long long)sPN[f0 - 1];
+      E1 += uniform64(sP1[f0 - 1]);
+      E2 += uniform64(sP2[f0 - 1]);
+      acc += (unsigned long long)__builtin_amdgcn_readfirstlane(sPN[f0 - 1]);
     }
     if (f0 < n) {
       // contested: one wave walks the rest by runs of acceptances and
       // refusals, 64 proposals at a time, with no block barriers
-      if (wv == 0) {
+      if (wv == 0 && !R.drive) {
+        if (R.axis == 1)
+          walk_window<1>(R, n, f0, sP1, sP2, sPN, sChg, sPos, sOld, links, E1, E2, acc);
+        else if (R.axis == 2)
+          walk_window<2>(R, n, f0, sP1, sP2, sPN, sChg, sPos, sOld, links, E1, E2, acc);
+        else
+          walk_window<3>(R, n, f0, sP1, sP2, sPN, sChg, sPos, sOld, links, E1, E2, acc);
+        if (lane == 0) {
+          sE[0] = E1;
+          sE[1] = E2;
+          sAcc = acc;
+        }
+      } else if (wv == 0) {
         for (uint32_t base = f0 & ~63u; base < n; base += 64) {
           const uint32_t cn = n - base < 64 ? n - base : 64;
           const uint32_t kk = base + (uint32_t)lane;
@@ -593,7 +729,7 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
             if (f >= cn) break;
             const unsigned long long gm = __ballot(chg && (uint32_t)lane > f && R(E1, E2, E1 + e1, E2 + e2));
             const uint32_t g = gm ? (uint32_t)__builtin_ctzll(gm) : cn;
-            if (chg && (uint32_t)lane >= f && (uint32_t)lane < g) links[sPos[kk]] = sOld[kk];
+            if (chg && (uint32_t)lane >= f && (uint32_t)lane < g) gmem(links)[sPos[kk]] = sOld[kk];
             s0 = g;
           }
         }
@@ -604,9 +740,9 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
         }
       }
       __syncthreads();
-      E1 = sE[0];
-      E2 = sE[1];
-      acc = sAcc;
+      E1 = uniform64(sE[0]);
+      E2 = uniform64(sE[1]);
+      acc = (unsigned long long)uniform64((long long)sAcc);
     }
     __syncthreads();  // the next segment overwrites the shared arrays
   }
