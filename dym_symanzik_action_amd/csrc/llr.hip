@@ -385,18 +385,20 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
 // does not move closer (refused).  Such proposals are settled apart and do
 // not break the runs below.
 //
-// One block of kDecide lanes per chain, a segment of kDecide proposals at a
-// time.  Block-wide inclusive prefix sums of the energy changes tell lane k
-// the running energy if every proposal from the first undecided one up to k
-// is accepted: the first lane whose rule fails ends a run of acceptances.
-// From there a refused proposal leaves the energy unchanged, so lane k's
-// energy if every proposal from the failing one up to k-1 is refused is
-// E + dE_k: the first lane whose rule holds on that ends a run of refusals,
-// whose lanes restore their links together.  This is the sequential rule
-// exactly: each decision sees the energy after all earlier accepted
-// proposals.  A segment costs one scan plus two block-wide minima per run.
-constexpr int kDecide = 1024;
+// One block of kDecide lanes per chain, a segment of kDecideSeg proposals at
+// a time, kDecidePer consecutive ones per lane.  Block-wide inclusive prefix
+// sums of the energy changes give every proposal the running energy if every
+// proposal from the segment's start up to it is accepted: the first one whose
+// rule fails ends the run of acceptances, found with one block-wide minimum.
+// From there one wave walks the rest of the segment by runs of acceptances
+// and refusals (a refused proposal leaves the energy unchanged), 64
+// proposals at a time, reading the prefix sums from LDS.  This is the
+// sequential rule exactly: each decision sees the energy after all earlier
+// accepted proposals.
+constexpr int kDecide = 512;
 constexpr int kDecideWaves = kDecide / 64;
+constexpr int kDecidePer = 8;
+constexpr int kDecideSeg = kDecide * kDecidePer;
 
 // Inclusive prefix sums over the wave: DPP row shifts (1, 2, 4, 8) scan each
 // row of 16 lanes, then the three lower rows' totals are added by row.
@@ -447,13 +449,15 @@ __device__ inline int scan_i32(int v, int lane) {
   return v + (row == 0 ? 0 : row == 1 ? t0 : row == 2 ? t0 + t1 : t0 + t1 + t2);
 }
 
-// The smallest lane index whose pred holds (or n).  Three slots rotate: the
-// slot of call k is reset after call k-2's readers have all passed call k-1's
-// barrier, and before call k's writers (they pass call k-1's barrier after
-// the reset in call k-1).
-__device__ inline uint32_t block_first(bool pred, uint32_t n, unsigned *slots, uint32_t &k) {
+// The smallest idx over the lanes whose pred holds (or n), for idx
+// increasing with the lane index.  Three slots rotate: the slot of call k is
+// reset after call k-2's readers have all passed call k-1's barrier, and
+// before call k's writers (they pass call k-1's barrier after the reset in
+// call k-1).
+__device__ inline uint32_t block_first(bool pred, uint32_t idx, uint32_t n, unsigned *slots, uint32_t &k) {
   const unsigned long long bm = __ballot(pred);
-  if ((threadIdx.x & 63u) == 0 && bm) atomicMin(&slots[k], (threadIdx.x & ~63u) + (unsigned)__builtin_ctzll(bm));
+  const unsigned cand = bm ? (unsigned)__builtin_amdgcn_readlane((int)idx, (int)__builtin_ctzll(bm)) : 0u;
+  if ((threadIdx.x & 63u) == 0 && bm) atomicMin(&slots[k], cand);
   __syncthreads();
   const unsigned r = (unsigned)__builtin_amdgcn_readfirstlane((int)slots[k]);
   const uint32_t nx = k == 2 ? 0 : k + 1;
@@ -465,16 +469,17 @@ __device__ inline uint32_t block_first(bool pred, uint32_t n, unsigned *slots, u
 // The contested walk under the window rule (every pass that is not a
 // drive-in pass): a changed proposal k is accepted iff the energy after it is
 // inside the window (axes 1, 2: the ruling energy; axis 3: both, the box),
-// i.e. (unsigned) (E + e_k - lo) <= hi - lo.  With q the inclusive prefix
-// sums of the changes, a run of acceptances from lane s, met with energy E_s,
-// has energy lo + o + q_k after lane k (o = E_s - lo - q_{s-1}), so it ends at
-// the first changed lane whose o + q_k leaves [0, hi - lo]: one add and one
-// compare per lane against uniform values.  A run of refusals from a failing
-// lane f leaves E unchanged and ends at the first changed lane after f whose
-// own change keeps E inside.  The walk follows the ruling energies; the other
-// energy and the hit count gain q_{f-1} - q_{s-1} per acceptance run (read
-// across lanes beside the walk's own reads); the lanes of the acceptance runs
-// are collected in a mask and the refused ones restore their links.
+// i.e. (unsigned) (E + e_k - lo) <= hi - lo.  With q the inclusive and x the
+// exclusive prefix sums of the changes, a run of acceptances from lane s, met
+// with energy E_s, has energy lo + o + q_k after lane k (o = E_s - lo - x_s),
+// so it ends at the first changed lane whose o + q_k leaves [0, hi - lo]: one
+// add and one compare per lane against uniform values.  A run of refusals
+// from a failing lane f leaves E unchanged and ends at the first changed lane
+// after f whose own change keeps E inside.  The walk follows the ruling
+// energies; the other energy and the hit count gain x_f - x_s per acceptance
+// run (read across lanes beside the walk's own reads); the lanes of the
+// acceptance runs are collected in a mask and the refused ones restore their
+// links.
 template <int AX>
 __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const long long *sP1, const long long *sP2,
                             const int *sPN, const uint8_t *sChg, const uint32_t *sPos, const uint16_t *sOld,
@@ -485,33 +490,29 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
   E1 = uniform64(E1);
   E2 = uniform64(E2);
   const unsigned long long Wa = (unsigned long long)(R.hi - R.lo), Wb = (unsigned long long)(R.hi2 - R.lo2);
-  for (uint32_t base = f0 & ~63u; base < n; base += 64) {  // n <= kDecide: base + 63 < kDecide
+  for (uint32_t base = f0 & ~63u; base < n; base += 64) {  // n <= kDecideSeg: base + 63 < kDecideSeg
     const uint32_t cn = n - base < 64 ? n - base : 64;
     const uint32_t kk = base + (uint32_t)lane, kp = kk ? kk - 1 : 0;
     const bool valid = (uint32_t)lane < cn;
-    // every load in bounds and unconditional, so that they wait together
-    const long long P1 = sP1[kk], P2 = sP2[kk], P1p = sP1[kp], P2p = sP2[kp];
-    const int qn = sPN[kk];
+    // every load in bounds and unconditional, so that they wait together:
+    // inclusive (q) and exclusive (x) prefix sums, lane 0's exclusive ones
+    // being the sums before the chunk
+    const long long q1 = sP1[kk], q2 = sP2[kk];
+    const long long x1 = kk ? sP1[kp] : 0, x2 = kk ? sP2[kp] : 0;
+    const int qn = sPN[kk], xn = kk ? sPN[kp] : 0;
     const uint8_t ch = sChg[kk];
-    const uint32_t kb = base ? base - 1 : 0;
-    const long long B1 = sP1[kb], B2 = sP2[kb];
-    const int BN = sPN[kb];
-    // before the chunk
-    const long long c1 = base ? uniform64(B1) : 0, c2 = base ? uniform64(B2) : 0;
-    const int cN = base ? __builtin_amdgcn_readfirstlane(BN) : 0;
-    const long long q1 = P1, q2 = P2;
-    const long long e1 = valid ? P1 - (kk ? P1p : 0) : 0, e2 = valid ? P2 - (kk ? P2p : 0) : 0;
     const unsigned long long cm = __ballot(valid & (ch != 0));  // changed lanes
     const uint32_t s_init = base < f0 ? f0 - base : 0;          // lanes before it are settled
     // the ruling energy: S1 on axes 1 and 3, S2 on axis 2; the box adds S2;
     // the other one (axes 1, 2) is summed by runs
-    const long long qa = AX == 2 ? q2 : q1, ea = AX == 2 ? e2 : e1, ca = AX == 2 ? c2 : c1;
-    const long long qo = AX == 2 ? q1 : q2, co = AX == 2 ? c1 : c2;
+    const long long qa = AX == 2 ? q2 : q1, xa_ = AX == 2 ? x2 : x1;
+    const long long qo = AX == 2 ? q1 : q2, xo = AX == 2 ? x1 : x2;
+    const long long ea = qa - xa_, eb = q2 - x2;
     long long so = 0;  // gain of the other energy over the chunk
     int sn = 0;        // hits of the accepted changed proposals
     uint32_t s = s_init;  // uniform over the wave, < 64
-    long long o = (AX == 2 ? E2 : E1) - R.lo - (s ? readlane64(qa, s - 1) : ca);
-    long long ob = AX == 3 ? E2 - R.lo2 - (s ? readlane64(q2, s - 1) : c2) : 0;
+    long long o = (AX == 2 ? E2 : E1) - R.lo - readlane64(xa_, s);
+    long long ob = AX == 3 ? E2 - R.lo2 - readlane64(x2, s) : 0;
     long long xa, xb = 0;  // the energies minus lo at the chunk's end
     unsigned long long amask = 0;
     for (;;) {
@@ -519,8 +520,8 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
       if (AX == 3) out = out | ((unsigned long long)(q2 + ob) > Wb);
       const unsigned long long from_s = ~0ull << s;
       const unsigned long long fm = __ballot(out) & cm & from_s;
-      const long long os = s ? readlane64(qo, s - 1) : co;  // before the run
-      const int ns = s ? __builtin_amdgcn_readlane(qn, (int)(s - 1)) : cN;
+      const long long os = readlane64(xo, s);  // before the run
+      const int ns = __builtin_amdgcn_readlane(xn, (int)s);
       if (!fm) {  // the rest of the chunk is accepted
         amask |= from_s;
         xa = o + readlane64(qa, cn - 1);
@@ -531,17 +532,17 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
       }
       const uint32_t f = (uint32_t)__builtin_ctzll(fm);
       amask |= from_s & ((1ull << f) - 1);
-      xa = o + (f ? readlane64(qa, f - 1) : ca);  // the energy met by lane f
-      if (AX == 3) xb = ob + (f ? readlane64(q2, f - 1) : c2);
-      if (AX != 3) so += (f ? readlane64(qo, f - 1) : co) - os;
-      sn += (f ? __builtin_amdgcn_readlane(qn, (int)(f - 1)) : cN) - ns;
+      xa = o + readlane64(xa_, f);  // the energy met by lane f
+      if (AX == 3) xb = ob + readlane64(x2, f);
+      if (AX != 3) so += readlane64(xo, f) - os;
+      sn += __builtin_amdgcn_readlane(xn, (int)f) - ns;
       bool in = (unsigned long long)(ea + xa) <= Wa;
-      if (AX == 3) in = in & ((unsigned long long)(e2 + xb) <= Wb);
+      if (AX == 3) in = in & ((unsigned long long)(eb + xb) <= Wb);
       const unsigned long long gm = __ballot(in) & cm & (f < 63 ? ~0ull << (f + 1) : 0ull);
       if (!gm) break;  // the rest of the chunk is refused: the energy stays
       const uint32_t g = (uint32_t)__builtin_ctzll(gm);
-      o = xa - readlane64(qa, g - 1);
-      if (AX == 3) ob = xb - readlane64(q2, g - 1);
+      o = xa - readlane64(xa_, g);
+      if (AX == 3) ob = xb - readlane64(x2, g);
       s = g;
     }
     const bool took = ((amask >> lane) & 1ull) != 0;
@@ -556,16 +557,17 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
 }
 
 __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
-  __shared__ long long sP1[kDecide], sP2[kDecide];
-  __shared__ int sPN[kDecide];
-  __shared__ uint32_t sPos[kDecide];
-  __shared__ uint16_t sOld[kDecide];
-  __shared__ uint8_t sChg[kDecide];
-  __shared__ long long sW1[kDecideWaves], sW2[kDecideWaves];
-  __shared__ int sWN[kDecideWaves], sWT[kDecideWaves];
+  __shared__ long long sP1[kDecideSeg], sP2[kDecideSeg];
+  __shared__ int sPN[kDecideSeg];
+  __shared__ uint32_t sPos[kDecideSeg];
+  __shared__ uint16_t sOld[kDecideSeg];
+  __shared__ uint8_t sChg[kDecideSeg];
+  __shared__ long long sW1[16], sW2[16];  // per wave; a row of 16 lanes reads them all
+  __shared__ int sWN[16], sWT[16];
   __shared__ unsigned slots[3];
   __shared__ long long sE[2];
   __shared__ unsigned long long sAcc;
+  static_assert(kDecideWaves <= 16, "the waves' totals fit one row of 16 lanes");
   const uint32_t c = blockIdx.x;
   const uint32_t tid = threadIdx.x;
   const int lane = (int)(tid & 63u), wv = (int)(tid >> 6);
@@ -578,36 +580,55 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
   unsigned long long acc = 0;
   uint16_t *links = LC.base.links;
   if (tid < 3) slots[tid] = 0xffffffffu;
+  if (tid < 16) {  // the slots past the last wave stay 0
+    sW1[tid] = 0;
+    sW2[tid] = 0;
+    sWN[tid] = 0;
+    sWT[tid] = 0;
+  }
   uint32_t k = 0;
-  // a segment's per-proposal inputs, loaded one segment ahead so that the
-  // next segment's loads are in flight while this one is decided
+  // a lane's kDecidePer consecutive proposals of a segment, loaded one
+  // segment ahead so that the next segment's loads are in flight while this
+  // one is decided; past the pass's end they are unchanged proposals with no
+  // hits
   struct In {
-    long long d1, d2;
-    int raw;
-    uint32_t pos;
-    uint16_t old;
+    long long d1[kDecidePer], d2[kDecidePer];
+    uint32_t pos[kDecidePer];
+    uint16_t old[kDecidePer];
+    uint8_t raw[kDecidePer];
   };
-  auto load_in = [&](uint32_t seg) {
-    In v{0, 0, 0, 0, 0};
-    if (seg < pr.count && tid < pr.count - seg) {
-      const uint32_t idx = pr.offset + seg + tid;
-      v.d1 = gmem(LC.dE1)[idx];
-      v.d2 = gmem(LC.dE2)[idx];
-      v.raw = (int)gmem(LC.nacc)[idx];
-      v.pos = gmem(LC.pos)[idx];
-      v.old = gmem(LC.old)[idx];
+  auto load_in = [&](uint32_t seg, In &v) {
+#pragma unroll
+    for (int i = 0; i < kDecidePer; ++i) {
+      const uint32_t j = seg + tid * (uint32_t)kDecidePer + (uint32_t)i;
+      const bool ok = j < pr.count;
+      const uint32_t idx = pr.offset + (ok ? j : 0u);
+      v.d1[i] = ok ? gmem(LC.dE1)[idx] : 0;
+      v.d2[i] = ok ? gmem(LC.dE2)[idx] : 0;
+      v.raw[i] = ok ? gmem(LC.nacc)[idx] : (uint8_t)0;
+      v.pos[i] = ok ? gmem(LC.pos)[idx] : 0u;
+      v.old[i] = ok ? gmem(LC.old)[idx] : (uint16_t)0;
     }
-    return v;
   };
-  In nxt = load_in(0);
-  for (uint32_t seg = 0; seg < pr.count; seg += kDecide) {
-    const uint32_t n = pr.count - seg < (uint32_t)kDecide ? pr.count - seg : (uint32_t)kDecide;
+  In nxt;
+  load_in(0, nxt);
+  for (uint32_t seg = 0; seg < pr.count; seg += kDecideSeg) {
+    const uint32_t n = pr.count - seg < (uint32_t)kDecideSeg ? pr.count - seg : (uint32_t)kDecideSeg;
     const In cur = nxt;
-    nxt = load_in(seg + kDecide);
-    const long long d1 = cur.d1, d2 = cur.d2;
-    const int raw = cur.raw;
-    const bool changed = (raw & kChanged) != 0;
-    const int dn = changed ? raw & ~kChanged : 0;  // hits of proposals that moved their link
+    if (seg + kDecideSeg < pr.count) load_in(seg + kDecideSeg, nxt);
+    bool chg[kDecidePer];
+    int dn[kDecidePer];  // hits of proposals that moved their link
+    long long l1 = 0, l2 = 0;
+    int ln = 0, lt = 0;  // lane totals; lt: hits of the unchanged proposals
+#pragma unroll
+    for (int i = 0; i < kDecidePer; ++i) {
+      chg[i] = (cur.raw[i] & kChanged) != 0;
+      dn[i] = chg[i] ? cur.raw[i] & ~kChanged : 0;
+      l1 += cur.d1[i];
+      l2 += cur.d2[i];
+      ln += dn[i];
+      lt += chg[i] ? 0 : cur.raw[i];
+    }
     if (R.drive && R.axis != 3) {
       // Drive-in on one window (axes 1, 2) with the energy E on one side of
       // the centre: a proposal gets strictly closer iff it moves toward the
@@ -619,10 +640,19 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
       // is settled with one block-wide sum instead of the walk below.
       const long long E = R.axis == 1 ? E1 : E2;
       const int sgn = E < R.mid ? 1 : (E > R.mid ? -1 : 0);  // uniform over the block
-      const long long dw = R.axis == 1 ? d1 : d2;
-      const bool toward = changed && (sgn > 0 ? dw > 0 : (sgn < 0 && dw < 0));
-      long long a1 = wave_sum_i64(toward ? d1 : 0), a2 = wave_sum_i64(toward ? d2 : 0);
-      const long long an = wave_sum_i64(toward ? dn : 0);
+      bool toward[kDecidePer];
+      long long a1 = 0, a2 = 0, an = 0;
+#pragma unroll
+      for (int i = 0; i < kDecidePer; ++i) {
+        const long long dw = R.axis == 1 ? cur.d1[i] : cur.d2[i];
+        toward[i] = chg[i] && (sgn > 0 ? dw > 0 : (sgn < 0 && dw < 0));
+        a1 += toward[i] ? cur.d1[i] : 0;
+        a2 += toward[i] ? cur.d2[i] : 0;
+        an += toward[i] ? dn[i] : 0;
+      }
+      a1 = wave_sum_i64(a1);
+      a2 = wave_sum_i64(a2);
+      an = wave_sum_i64(an);
       if (lane == 0) {
         sW1[wv] = a1;
         sW2[wv] = a2;
@@ -640,47 +670,62 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
       __syncthreads();  // the wave slots are written again below
       const long long Eend = E + (R.axis == 1 ? a1 : a2);
       if (sgn != 0 && (sgn > 0 ? Eend <= R.mid : Eend >= R.mid)) {
-        if (changed && !toward) gmem(links)[cur.pos] = cur.old;
+#pragma unroll
+        for (int i = 0; i < kDecidePer; ++i)
+          if (chg[i] && !toward[i]) gmem(links)[cur.pos[i]] = cur.old[i];
         E1 += a1;
         E2 += a2;
         acc += (unsigned long long)nt;
         continue;
       }
     }
-    long long p1 = scan_i64(d1, lane), p2 = scan_i64(d2, lane);
-    int pn = scan_i32(dn, lane);
-    const int pt = scan_i32(changed ? 0 : raw, lane);  // hits of the others
+    // block-wide prefix sums: the lane's own inclusive ones plus the totals
+    // of the lanes below it (in the wave, then of the waves below)
+    const long long w1 = scan_i64(l1, lane), w2 = scan_i64(l2, lane);
+    const int wn = scan_i32(ln, lane), wt = scan_i32(lt, lane);
     if (lane == 63) {
-      sW1[wv] = p1;
-      sW2[wv] = p2;
-      sWN[wv] = pn;
-      sWT[wv] = pt;
+      sW1[wv] = w1;
+      sW2[wv] = w2;
+      sWN[wv] = wn;
+      sWT[wv] = wt;
     }
-    sOld[tid] = cur.old;
-    sPos[tid] = cur.pos;
-    sChg[tid] = changed ? 1 : 0;
     __syncthreads();
+    long long b1 = w1 - l1, b2 = w2 - l2;
+    int bn = wn - ln;
     {
-      // the waves' totals, one per lane of each row of 16, scanned along the
-      // row: lane w - 1 holds the sum over the waves below wave w
-      static_assert(kDecideWaves == 16, "one row of 16 lanes per wave's totals");
       const int wl = lane & 15;
       const long long t1 = row_scan_i64(sW1[wl]), t2 = row_scan_i64(sW2[wl]);
       const int tn = row_scan_i32(sWN[wl]), tt = row_scan_i32(sWT[wl]);
       const uint32_t wu = (uint32_t)__builtin_amdgcn_readfirstlane(wv);
       if (wu > 0) {
-        p1 += readlane64(t1, wu - 1);
-        p2 += readlane64(t2, wu - 1);
-        pn += __builtin_amdgcn_readlane(tn, (int)(wu - 1));
+        b1 += readlane64(t1, wu - 1);
+        b2 += readlane64(t2, wu - 1);
+        bn += __builtin_amdgcn_readlane(tn, (int)(wu - 1));
       }
-      if (!R.drive) acc += (unsigned long long)(unsigned)__builtin_amdgcn_readlane(tt, 15);  // unchanged links: all accepted under the window rule
+      // unchanged proposals: all accepted under the window rule
+      if (!R.drive) acc += (unsigned long long)(unsigned)__builtin_amdgcn_readlane(tt, 15);
     }
-    sP1[tid] = p1;
-    sP2[tid] = p2;
-    sPN[tid] = pn;
-    __syncthreads();
-    // the whole segment at once: the first proposal whose rule fails
-    const uint32_t f0 = block_first(changed && !R(E1 + p1 - d1, E2 + p2 - d2, E1 + p1, E2 + p2), n, slots, k);
+    // every proposal's prefix sums into LDS, and the lane's first proposal
+    // whose rule fails if all from the segment's start up to it are accepted
+    uint32_t fl = kDecidePer;
+#pragma unroll
+    for (int i = 0; i < kDecidePer; ++i) {
+      b1 += cur.d1[i];
+      b2 += cur.d2[i];
+      bn += dn[i];
+      const uint32_t j = tid * (uint32_t)kDecidePer + (uint32_t)i;
+      sP1[j] = b1;
+      sP2[j] = b2;
+      sPN[j] = bn;
+      sPos[j] = cur.pos[i];
+      sOld[j] = cur.old[i];
+      sChg[j] = chg[i] ? 1 : 0;
+      if (fl == kDecidePer && chg[i] && !R(E1 + b1 - cur.d1[i], E2 + b2 - cur.d2[i], E1 + b1, E2 + b2))
+        fl = (uint32_t)i;
+    }
+    // the whole segment at once: the first proposal whose rule fails (the
+    // barrier inside also publishes the stores above)
+    const uint32_t f0 = block_first(fl < (uint32_t)kDecidePer, tid * (uint32_t)kDecidePer + fl, n, slots, k);
     if (f0 > 0) {
       E1 += uniform64(sP1[f0 - 1]);
       E2 += uniform64(sP2[f0 - 1]);
