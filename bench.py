@@ -62,7 +62,7 @@ GATHERS_CLASS = 85      # the 24 staple-times-link products, built directly; per
 GATHERS_CLASS_ACCEPT = 25   # new link + 24 staple products moved by the accepted factor
 GATHERS_CLASS_EXACT = 98    # b', b^-1 and 4 per staple for an exact re-decision
 KERNEL_NAME = {"table": "k_sweep<4>", "class": "k_sweep_cls<4, true>", "class-narrow": "k_sweep_cls<4, false>",
-               "matrix": "k_sweep_mat<4>"}
+               "matrix": "k_sweep_mat<4>", "field": "k_sweep_cls<4, false, false, true>"}
 HBM_PEAK_GBS = 8000.0
 L2_REQ_PEAK = 16 * 8 * 2.4e9   # L2 requests/s: 16 channels per XCD, 8 XCDs, 2.4 GHz
 PMC_FILE = {"table": "k_sweep_32x64_pmc.txt", "class-narrow": "k_sweep_cls_narrow_32x64_pmc.txt"}
@@ -298,7 +298,7 @@ def main():
                     help="HBM budget of the serving engine in MiB (admission by footprint)")
     ap.add_argument("--serve-serial-every", type=int, default=5,
                     help="time every n-th request alone for the one-at-a-time rate")
-    ap.add_argument("--kernel", choices=["auto", "table", "class", "class-narrow", "matrix"], default="auto",
+    ap.add_argument("--kernel", choices=["auto", "table", "class", "class-narrow", "matrix", "field"], default="auto",
                     help="sweep kernel (identical results): element values via L2 gathers, or class values in LDS")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run a plain copy kernel of this size (scale for the FETCH/WRITE_SIZE counters)")
@@ -355,6 +355,11 @@ def main():
         # evaluates them (98)
         gathers_per_unit = GATHERS_CLASS + ((acc1 - acc0) + (GATHERS_CLASS + GATHERS_CLASS_EXACT) *
                                             (ex1 - ex0)) / (args.steps * LINKS)
+    elif form == "field":
+        # the staple-times-link products are GF(4) matrix products in
+        # registers: table gathers only for accepted hits and re-decisions
+        gathers_per_unit = (GATHERS_CLASS_ACCEPT * (acc1 - acc0) +
+                            GATHERS_CLASS_EXACT * (ex1 - ex0)) / (args.steps * LINKS)
     else:
         gathers_per_unit = GATHERS_CLASS + (GATHERS_CLASS_ACCEPT * (acc1 - acc0) +
                                             GATHERS_CLASS_EXACT * (ex1 - ex0)) / (args.steps * LINKS)

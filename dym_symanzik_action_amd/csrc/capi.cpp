@@ -13,6 +13,7 @@
 
 #include "../../include/dsa.h"
 #include "dsa_math.hpp"
+#include "gf4.hpp"
 #include "group.hpp"
 #include "internal.hpp"
 #include "kernels.hpp"
@@ -67,6 +68,7 @@ struct dsa_chain {
   int kernel = DSA_KERNEL_AUTO;
   int cls_best = dsa::kClsNone;
   bool narrow_ok = false;
+  bool field_ok = false;  // the narrow form with GF(4) products fits here
   size_t lds_max = 0;  // LDS bytes per block on the chain's device
   double margin8 = 0, margin16 = 0, margin_scale = 1;
   // the matrix kernel: runs here (the group has matrices, they fit LDS),
@@ -98,6 +100,9 @@ dsa::Tables dsa::tables_view(const dsa_group *g, const DeviceTables &t) {
   T.cval = t.cval;
   T.cval0 = t.cval0;
   T.mat = t.mat;
+  T.fmat = t.fmat;
+  T.finv = t.finv;
+  T.fkey = t.fkey;
   return T;
 }
 
@@ -222,7 +227,8 @@ namespace {
 
 void free_tables(DeviceTables &t) {
   for (void *p : {(void *)t.mult, (void *)t.inv, (void *)t.pool, (void *)t.retr, (void *)t.imtr, (void *)t.cls,
-                  (void *)t.cprod, (void *)t.rcq, (void *)t.mdq, (void *)t.cval, (void *)t.cval0, (void *)t.mat})
+                  (void *)t.cprod, (void *)t.rcq, (void *)t.mdq, (void *)t.cval, (void *)t.cval0, (void *)t.mat,
+                  (void *)t.fmat, (void *)t.finv, (void *)t.fkey})
     if (p) (void)hipFree(p);
   t = DeviceTables();
 }
@@ -265,6 +271,16 @@ int fill_tables(const dsa::Group &G, dsa_group *g, DeviceTables &t) {
     HIP_TRY(hipMemcpy(t.cval, G.cval.data(), G.cval.size() * sizeof(int16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(t.cval0, G.cval0.data(), P * sizeof(int16_t), hipMemcpyHostToDevice));
   }
+  if (!G.fmat.empty()) {
+    std::vector<uint32_t> fi(P);
+    for (size_t e = 0; e < P; ++e) fi[e] = G.fmat[G.inv[e]];
+    HIP_TRY(hipMalloc(&t.fmat, P * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&t.finv, P * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&t.fkey, G.fkey.size() * sizeof(uint16_t)));
+    HIP_TRY(hipMemcpy(t.fmat, G.fmat.data(), P * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.finv, fi.data(), P * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(t.fkey, G.fkey.data(), G.fkey.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  }
   if (g->mat_state == 0) g->mat_state = dsa::su3_check(G, g->mat32, g->mat_dev) && g->mat_dev < 1e-9 ? 1 : -1;
   if (g->mat_state > 0) {
     HIP_TRY(hipMalloc(&t.mat, g->mat32.size() * sizeof(float)));
@@ -297,13 +313,17 @@ namespace {
 // The class form a chain's large passes run (kClsNone: the table kernel).
 // CLASS takes the int16-value form where it fits, else the byte-index one;
 // AUTO prefers the byte-index form (half the LDS of the int16 one), which
-// both certifies more decisions and measures faster on S(1080).
+// both certifies more decisions and measures faster on S(1080), with its
+// products formed in the group's GF(4) form where the group has one.
 int cls_mode(const dsa_chain *c) {
   switch (c->kernel) {
     case DSA_KERNEL_CLASS: return c->cls_best;
     case DSA_KERNEL_CLASS_NARROW: return dsa::kClsNarrow;
     case DSA_KERNEL_MATRIX: return dsa::kClsMatrix;
-    case DSA_KERNEL_AUTO: return c->narrow_ok && c->margin8 < 1e-3 ? (int)dsa::kClsNarrow : c->cls_best;
+    case DSA_KERNEL_FIELD: return dsa::kClsField;
+    case DSA_KERNEL_AUTO:
+      if (c->narrow_ok && c->margin8 < 1e-3) return c->field_ok ? (int)dsa::kClsField : (int)dsa::kClsNarrow;
+      return c->cls_best;
     default: return dsa::kClsNone;
   }
 }
@@ -460,6 +480,7 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
     c->margin16 = dsa::class_margin(g->g, (int)p->D, p->beta0, p->beta1, p->beta2, 0x1p-14);
     // + the kernels' static class-value array
     c->narrow_ok = dsa::sweep_cls_lds_bytes(c->T, dsa::kClsNarrow) + 256 <= (size_t)lds_max;
+    c->field_ok = c->T.fmat && dsa::sweep_cls_lds_bytes(c->T, dsa::kClsField) + 256 <= (size_t)lds_max;
     const bool wide_ok = c->T.cval && dsa::sweep_cls_lds_bytes(c->T, dsa::kClsWide) + 256 <= (size_t)lds_max;
     if (wide_ok && c->margin16 < 1e-2)
       c->cls_best = dsa::kClsWide;
@@ -524,6 +545,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.cmargin = (mode == dsa::kClsMatrix ? c->mat_m0 : mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
   A.cmargin_acc = c->mat_macc * c->margin_scale;
   A.bound = c->bound_next ? 1 : 0;
+  A.field_all = c->kernel == DSA_KERNEL_FIELD ? 1 : 0;
   const size_t np = c->S.passes.size();
   if (c->profiling == 1 && c->pev.size() < 2 * np) {
     for (size_t q = c->pev.size(); q < 2 * np; ++q) {
@@ -581,6 +603,7 @@ int dsa_chain_kernel_form(dsa_chain *c, int32_t *form) {
   if (!c || !form) return fail(DSA_E_ARG, "dsa_chain_kernel_form: NULL argument");
   const int mode = cls_mode(c);
   *form = mode == dsa::kClsMatrix  ? DSA_KERNEL_MATRIX
+          : mode == dsa::kClsField ? DSA_KERNEL_FIELD
           : mode == dsa::kClsWide  ? DSA_KERNEL_CLASS
           : mode == dsa::kClsNarrow ? DSA_KERNEL_CLASS_NARROW
                                     : DSA_KERNEL_TABLE;
@@ -590,7 +613,7 @@ int dsa_chain_kernel_form(dsa_chain *c, int32_t *form) {
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale) {
   if (!c) return fail(DSA_E_ARG, "dsa_chain_set_kernel: NULL chain");
   if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS &&
-      kernel != DSA_KERNEL_CLASS_NARROW && kernel != DSA_KERNEL_MATRIX)
+      kernel != DSA_KERNEL_CLASS_NARROW && kernel != DSA_KERNEL_MATRIX && kernel != DSA_KERNEL_FIELD)
     return fail(DSA_E_ARG, "dsa_chain_set_kernel: unknown kernel " + std::to_string(kernel));
   if (kernel == DSA_KERNEL_MATRIX && !c->mat_ok)
     return fail(DSA_E_STATE, "dsa_chain_set_kernel: the matrix kernel needs the group's SU(3) matrices (appended to "
@@ -602,6 +625,9 @@ int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale) {
                              "more than 63 classes, or class tables beyond the LDS)");
   if (kernel == DSA_KERNEL_CLASS_NARROW && !c->narrow_ok)
     return fail(DSA_E_STATE, "dsa_chain_set_kernel: the narrow class tables do not fit this device's LDS");
+  if (kernel == DSA_KERNEL_FIELD && !(c->narrow_ok && c->field_ok))
+    return fail(DSA_E_STATE, "dsa_chain_set_kernel: the field kernel needs the narrow class tables and a GF(4) "
+                             "form of the group (a subgroup of SL(3,4)) in this device's LDS");
   c->kernel = kernel;
   c->margin_scale = margin_scale;
   return 0;
@@ -733,6 +759,25 @@ int dsa_group_matrices(const dsa_group *g, double *out) {
   if (!g || !out) return fail(DSA_E_ARG, "dsa_group_matrices: NULL argument");
   if (g->g.su3.empty()) return fail(DSA_E_STATE, "group file " + g->g.path + " has no appended SU(3) matrices");
   memcpy(out, g->g.su3.data(), g->g.su3.size() * sizeof(double));
+  return 0;
+}
+
+int dsa_group_field(const dsa_group *g, uint32_t *mat, uint16_t *key) {
+  if (!g) return fail(DSA_E_ARG, "dsa_group_field: NULL group");
+  if (g->g.fmat.empty()) return fail(DSA_E_STATE, "group " + g->g.path + " has no GF(4) form");
+  const size_t P = g->g.P;
+  // packed planes (lo bits 0-8, hi bits 20-28, entry (r, c) at 3r + c) to
+  // 2-bit entries; the first two rows of the plain code are its low 12 bits
+  // and key the same element as the packed word's key
+  std::vector<uint16_t> k2(dsa::kGf4Keys, 0xFFFF);
+  for (size_t e = 0; e < P; ++e) {
+    const uint32_t w = g->g.fmat[e];
+    uint32_t plain = 0;
+    for (int q = 0; q < 9; ++q) plain |= (((w >> q) & 1u) | ((w >> (20 + q)) & 1u) << 1) << (2 * q);
+    if (mat) mat[e] = plain;
+    k2[plain & 0xFFFu] = (uint16_t)e;
+  }
+  if (key) memcpy(key, k2.data(), k2.size() * sizeof(uint16_t));
   return 0;
 }
 

@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "dsa_math.hpp"
+#include "gf4.hpp"
 #include "kernels.hpp"
 
 namespace dsa {
@@ -231,6 +232,72 @@ __device__ inline void build_t(const uint16_t *links, const Geom &G, const TabVi
     t[NS + 6 * j + 3] = T.M(T.M(p3, q3), Bd);
     t[NS + 6 * j + 4] = T.M(T.M(p4, q4), Bd);
     t[NS + 6 * j + 5] = T.M(T.M(Y, q5), C);
+  }
+}
+
+// Group products from the GF(4) form (gf4.hpp) instead of the table: the
+// packed matrix of an element and of its inverse, and the element of a
+// packed matrix, all in LDS.
+struct FieldView {
+  const uint32_t *m;    // P
+  const uint32_t *mi;   // P: matrix of the inverse
+  const uint16_t *key;  // kGf4Keys
+  __device__ inline uint32_t M(uint32_t e) const { return m[e]; }
+  __device__ inline uint32_t Mi(uint32_t e) const { return mi[e]; }
+  __device__ inline uint32_t E(uint32_t w) const { return key[gf4_key(w)]; }
+};
+
+// build_t with every product a packed GF(4) matrix product (no memory
+// access; the same 85 products): the links' matrices and their inverses'
+// come from LDS, and each t_k's element from its key.
+template <int D>
+__device__ inline void build_t_field(const uint16_t *links, const Geom &G, const FieldView &F, const Site<D> &s,
+                                     int mu, uint32_t b, uint32_t *t) {
+  constexpr int NS = 2 * (D - 1);
+  const uint32_t Mb = F.M(b);
+  const uint32_t umu_p = F.M(U<D>(links, G, s, mu, mu, 1, mu, 0));             // U_mu(x+mu)
+  const uint32_t C = gf4_mul3(F.M(U<D>(links, G, s, mu, mu, -1, mu, 0)), Mb);  // U_mu(x-mu) b
+#pragma unroll
+  for (int j = 0; j < D - 1; ++j) {
+    const int nu = j + (j >= mu ? 1 : 0);
+    const uint32_t n_10 = F.M(U<D>(links, G, s, nu, mu, 1, nu, 0));
+    const uint32_t n_00i = F.Mi(U<D>(links, G, s, nu, mu, 0, nu, 0));
+    const uint32_t n_1mi = F.Mi(U<D>(links, G, s, nu, mu, 1, nu, -1));
+    const uint32_t n_0m = F.M(U<D>(links, G, s, nu, mu, 0, nu, -1));
+    const uint32_t n_11 = F.M(U<D>(links, G, s, nu, mu, 1, nu, 1));
+    const uint32_t n_01i = F.Mi(U<D>(links, G, s, nu, mu, 0, nu, 1));
+    const uint32_t n_20 = F.M(U<D>(links, G, s, nu, mu, 2, nu, 0));
+    const uint32_t n_m0i = F.Mi(U<D>(links, G, s, nu, mu, -1, nu, 0));
+    const uint32_t n_1Mi = F.Mi(U<D>(links, G, s, nu, mu, 1, nu, -2));
+    const uint32_t n_0M = F.M(U<D>(links, G, s, nu, mu, 0, nu, -2));
+    const uint32_t n_2mi = F.Mi(U<D>(links, G, s, nu, mu, 2, nu, -1));
+    const uint32_t n_mm = F.M(U<D>(links, G, s, nu, mu, -1, nu, -1));
+    const uint32_t m_01i = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, 1));
+    const uint32_t m_0mi = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, -1));
+    const uint32_t m_02i = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, 2));
+    const uint32_t m_11i = F.Mi(U<D>(links, G, s, mu, mu, 1, nu, 1));
+    const uint32_t m_m1i = F.Mi(U<D>(links, G, s, mu, mu, -1, nu, 1));
+    const uint32_t m_0Mi = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, -2));
+    const uint32_t m_1mi = F.Mi(U<D>(links, G, s, mu, mu, 1, nu, -1));
+    const uint32_t m_mmi = F.Mi(U<D>(links, G, s, mu, mu, -1, nu, -1));
+    const uint32_t A = gf4_mul3(n_00i, Mb);
+    const uint32_t Bd = gf4_mul3(n_0m, Mb);
+    const uint32_t X = gf4_mul3(n_10, m_01i);
+    const uint32_t Y = gf4_mul3(n_1mi, m_0mi);
+    const uint32_t p0 = gf4_mul3(n_10, n_11), q0 = gf4_mul3(m_02i, n_01i);
+    const uint32_t p1 = gf4_mul3(umu_p, n_20), q1 = gf4_mul3(m_11i, m_01i);
+    const uint32_t q2 = gf4_mul3(m_m1i, n_m0i);
+    const uint32_t p3 = gf4_mul3(n_1mi, n_1Mi), q3 = gf4_mul3(m_0Mi, n_0M);
+    const uint32_t p4 = gf4_mul3(umu_p, n_2mi), q4 = gf4_mul3(m_1mi, m_0mi);
+    const uint32_t q5 = gf4_mul3(m_mmi, n_mm);
+    t[2 * j] = F.E(gf4_mul3(X, A));
+    t[2 * j + 1] = F.E(gf4_mul3(Y, Bd));
+    t[NS + 6 * j + 0] = F.E(gf4_mul3(gf4_mul3(p0, q0), A));
+    t[NS + 6 * j + 1] = F.E(gf4_mul3(gf4_mul3(p1, q1), A));
+    t[NS + 6 * j + 2] = F.E(gf4_mul3(gf4_mul3(X, q2), C));
+    t[NS + 6 * j + 3] = F.E(gf4_mul3(gf4_mul3(p3, q3), Bd));
+    t[NS + 6 * j + 4] = F.E(gf4_mul3(gf4_mul3(p4, q4), Bd));
+    t[NS + 6 * j + 5] = F.E(gf4_mul3(gf4_mul3(Y, q5), C));
   }
 }
 
@@ -505,6 +572,19 @@ __device__ inline void cls_prepare(ClsLink<D> &S, const LinkCtx &C, const L &Lk,
   S.lp = (size_t)mu * G.V + lpos(G, i, S.s.par);
   S.b = S.b0 = gmem(C.links)[S.lp];
   build_t<D>(C.links, G, T, S.s, mu, S.b, S.t);
+  cls_sums_bound<D, BOUND>(Lk, S);
+  S.ms = C.rng == kRngMinstd ? gmem(C.mstate)[i] : 0u;
+}
+
+// cls_prepare with the products from the GF(4) form.
+template <int D, bool BOUND, class L>
+__device__ inline void cls_prepare_field(ClsLink<D> &S, const LinkCtx &C, const L &Lk, const Geom &G,
+                                         const FieldView &F, int mu, uint32_t i) {
+  S.i = i;
+  S.s = decode<D>(G, i);
+  S.lp = (size_t)mu * G.V + lpos(G, i, S.s.par);
+  S.b = S.b0 = gmem(C.links)[S.lp];
+  build_t_field<D>(C.links, G, F, S.s, mu, S.b, S.t);
   cls_sums_bound<D, BOUND>(Lk, S);
   S.ms = C.rng == kRngMinstd ? gmem(C.mstate)[i] : 0u;
 }

@@ -110,6 +110,7 @@ std::string load_group(const char *path, Group &g) {
   g.nn_retr = nn_retr;
   if (g.pool.empty()) return "group has no proposal elements (all ReTr equal)";
   build_classes(g);
+  build_field(g);
   return std::string();
 }
 

@@ -42,6 +42,12 @@ struct Group {
   std::vector<int32_t> mdq;
   std::vector<int16_t> cval, cval0;
   double dmax = 0, rmax = 0;
+  // A faithful representation by 3x3 matrices over GF(4) (empty where none
+  // was found): fmat[e] is e's matrix in the packed form of gf4.hpp, and
+  // fkey[gf4_key(fmat[e])] = e (0xFFFF for keys of no element), so that
+  // e f = fkey[gf4_key(gf4_mul3(fmat[e], fmat[f]))] for every pair.
+  std::vector<uint32_t> fmat;
+  std::vector<uint16_t> fkey;
 };
 
 constexpr unsigned kMaxClassOrder = 8192;
@@ -58,5 +64,10 @@ std::string load_group(const char *path, Group &g);
 
 // Fills cls, cprod, rc, dmax, rmax (left empty when not applicable).
 void build_classes(Group &g);
+
+// Fills fmat and fkey when the group embeds in SL(3,4) and is generated by
+// two elements; every product of the table is checked against them, and
+// both stay empty otherwise.
+void build_field(Group &g);
 
 }  // namespace dsa

@@ -23,6 +23,8 @@ struct DeviceTables {
   int32_t *rcq = nullptr, *mdq = nullptr;
   int16_t *cval = nullptr, *cval0 = nullptr;
   float *mat = nullptr;  // FP32 SU(3) matrices, 20 floats per element (null if none)
+  uint32_t *fmat = nullptr, *finv = nullptr;  // GF(4) form (null if none)
+  uint16_t *fkey = nullptr;
 };
 
 }  // namespace dsa

@@ -50,8 +50,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs A) {
 // moves still gather from the global table.
 constexpr int kClsBlockNarrow = 512, kClsBlockWide = 1024;
 
-template <int D, bool W, bool B>
-__global__ __launch_bounds__(W ? kClsBlockWide : kClsBlockNarrow) __attribute__((amdgpu_waves_per_eu(4, 8)))
+// F (narrow only): the staple-times-link products from the group's GF(4)
+// form instead of table gathers; its tables (16.6 KB for S(1080)) push the
+// block's LDS past half a CU's, so one 1024-lane block per CU.
+template <int D, bool W, bool B, bool F = false>
+__global__ __launch_bounds__(W || F ? kClsBlockWide : kClsBlockNarrow) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_sweep_cls(SweepArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   // class values at a fixed LDS address: the narrow table's bytes index them
@@ -63,6 +66,11 @@ void k_sweep_cls(SweepArgs A) {
   unsigned char *s_tab = s_tab0 + pad16((W ? 2 : 1) * (size_t)P);
   // narrow form: the per-class hit bounds after the class table (null if none)
   int32_t *s_md = !W && A.T.mdq ? reinterpret_cast<int32_t *>(s_tab + pad16((size_t)np * P)) : nullptr;
+  // field form: the packed matrices, their inverses' and the key table next
+  unsigned char *s_f = s_tab + pad16((size_t)np * P) + (A.T.mdq ? pad16(4 * (size_t)A.T.ncls) : 0);
+  uint32_t *s_fm = reinterpret_cast<uint32_t *>(s_f);
+  uint32_t *s_fmi = reinterpret_cast<uint32_t *>(s_f + pad16(4 * (size_t)P));
+  uint16_t *s_fkey = reinterpret_cast<uint16_t *>(s_f + 2 * pad16(4 * (size_t)P));
   if constexpr (W) {
     // 1024 lanes: every table in at most 10 rounds
     Staged<2, uint16_t> g_inv;
@@ -79,6 +87,19 @@ void k_sweep_cls(SweepArgs A) {
     g_pool.store(s_pool, A.T.pool, np);
     g_tab0.store(reinterpret_cast<u32x4 *>(s_tab0), c0, n0);
     g_tab.store(reinterpret_cast<u32x4 *>(s_tab), c1, n1);
+  } else if constexpr (F) {
+    constexpr uint32_t nk = kGf4Keys * sizeof(uint16_t) / 16;
+    const u32x4 *ks = reinterpret_cast<const u32x4 *>(A.T.fkey);
+    Staged<2, uint32_t> fm, fmi;
+    Staged<1, u32x4> fk;
+    fm.load(A.T.fmat, P);
+    fmi.load(A.T.finv, P);
+    fk.load(ks, nk);
+    ClsNarrowStage g;
+    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
+    fm.store(s_fm, A.T.fmat, P);
+    fmi.store(s_fmi, A.T.finv, P);
+    fk.store(reinterpret_cast<u32x4 *>(s_fkey), ks, nk);
   } else {
     ClsNarrowStage g;
     g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
@@ -99,8 +120,17 @@ void k_sweep_cls(SweepArgs A) {
   unsigned long long acc = 0;
   uint32_t nfall = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride)
-    acc += update_link_cls<D, B>(C, Lk, K, A.G, T, A.mu, A.sites[k], nfall);
+  if constexpr (F) {
+    const FieldView FV{s_fm, s_fmi, s_fkey};
+    for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride) {
+      ClsLink<D> S;
+      cls_prepare_field<D, B>(S, C, Lk, A.G, FV, A.mu, A.sites[k]);
+      acc += cls_hits<D, B>(S, C, Lk, K, T, A.mu, nfall, nullptr);
+    }
+  } else {
+    for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride)
+      acc += update_link_cls<D, B>(C, Lk, K, A.G, T, A.mu, A.sites[k], nfall);
+  }
   acc = wave_sum_u64(acc);
   const unsigned long long nf = wave_sum_u64(nfall);
   if ((threadIdx.x & 63u) == 0) {
@@ -542,6 +572,8 @@ size_t sweep_lds_bytes(const Tables &T) {
 }
 
 size_t sweep_cls_lds_bytes(const Tables &T, int mode) {
+  if (mode == kClsField)
+    return sweep_cls_lds_bytes(T, kClsNarrow) + 2 * pad16(4 * (size_t)T.P) + kGf4Keys * sizeof(uint16_t);
   if (mode == kClsMatrix) return 80 * (size_t)T.P + pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool);
   if (mode == kClsWide)
     return pad16(2 * (size_t)T.P) + pad16(2 * (size_t)T.npool) + pad16(2 * (size_t)T.P) +
@@ -555,7 +587,8 @@ hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
   const size_t lds_small = sweep_cls_lds_bytes(A.T, kClsNarrow);
   const bool sm = A.count < kClsSmallPass ||
                   lds_small + cls_small_static_lds(256) > A.lds_max;  // 128 links where 256 do not fit
-  if (A.use_cls == kClsNarrow && A.rng == kRngPhilox && A.count < kClsMidPass &&
+  if ((A.use_cls == kClsNarrow || (A.use_cls == kClsField && !A.field_all)) && A.rng == kRngPhilox &&
+      A.count < kClsMidPass &&
       lds_small + cls_small_static_lds(sm ? 128 : 256) <= A.lds_max) {
     if (!A.fall || !A.T.cprod) return hipErrorInvalidValue;
     const uint32_t nl = sm ? 128 : 256, nb = (A.count + nl - 1) / nl;
@@ -594,6 +627,27 @@ hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
       case 5: hipLaunchKernelGGL(k_sweep_mat<5>, dim3(nb), dim3(kMatBlock), lds, s, A); break;
       default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+  }
+  if (A.use_cls == kClsField) {
+    if (!A.fall || !A.T.cprod || !A.T.fmat) return hipErrorInvalidValue;
+    uint32_t nb = (A.count + kClsBlockWide - 1) / kClsBlockWide;
+    if (max_blocks > 0 && nb > (uint32_t)max_blocks) nb = (uint32_t)max_blocks;
+    const size_t lds = sweep_cls_lds_bytes(A.T, kClsField);
+    const bool bnd = A.bound && A.T.mdq;
+#define DSA_FIELD_CASE(DD)                                                                                    \
+  case DD:                                                                                                    \
+    if (bnd) hipLaunchKernelGGL((k_sweep_cls<DD, false, true, true>), dim3(nb), dim3(kClsBlockWide), lds, s, A); \
+    else hipLaunchKernelGGL((k_sweep_cls<DD, false, false, true>), dim3(nb), dim3(kClsBlockWide), lds, s, A);   \
+    break;
+    switch (A.G.D) {
+      DSA_FIELD_CASE(2)
+      DSA_FIELD_CASE(3)
+      DSA_FIELD_CASE(4)
+      DSA_FIELD_CASE(5)
+      default: return hipErrorInvalidValue;
+    }
+#undef DSA_FIELD_CASE
     return hipGetLastError();
   }
   if (A.use_cls == kClsNarrow || A.use_cls == kClsWide) {

@@ -38,6 +38,10 @@ struct Tables {
   // the group's SU(3) matrices in FP32, 20 floats per element (re[9], im[9],
   // 2 unused), row-major (null where the group file has none)
   const float *mat;
+  // the group's GF(4) form (gf4.hpp; null where it has none): packed matrix
+  // of each element and of its inverse, and the element of each key
+  const uint32_t *fmat, *finv;
+  const uint16_t *fkey;
 };
 
 enum RngKind : int { kRngMinstd = 0, kRngPhilox = 1 };
@@ -72,10 +76,15 @@ struct SweepArgs {
   // the device's LDS bytes per block: the small-pass class forms run only
   // where their draw arrays fit beside the tables
   size_t lds_max;
+  // kClsField: every pass through the field kernel (else passes below the
+  // small-pass sizes take the narrow small-pass kernel)
+  int field_all;
 };
 
 size_t sweep_lds_bytes(const Tables &T);
-enum ClsMode : int { kClsNone = 0, kClsNarrow = 1, kClsWide = 2, kClsMatrix = 3 };
+// kClsField: the narrow class update with the staple-times-link products
+// formed from the group's GF(4) matrices (no table gathers for them)
+enum ClsMode : int { kClsNone = 0, kClsNarrow = 1, kClsWide = 2, kClsMatrix = 3, kClsField = 4 };
 size_t sweep_cls_lds_bytes(const Tables &T, int mode);
 __host__ __device__ inline size_t pad16(size_t n) { return (n + 15) & ~(size_t)15; }
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s);

@@ -22,7 +22,7 @@ DSA_RNG = {"philox": 0, "minstd": 1}
 DSA_ORDER = {"colour": 0, "reference": 1}
 DSA_START = {"hot": 0, "cold": 1}
 DSA_SUM = {"tree": 0, "ordered": 1}
-DSA_KERNEL = {"auto": 0, "table": 1, "class": 2, "class-narrow": 3, "matrix": 4}
+DSA_KERNEL = {"auto": 0, "table": 1, "class": 2, "class-narrow": 3, "matrix": 4, "field": 5}
 LOOP_KIND = {"plaq": 0, "rect": 1, "poly": 2, "wilson": 3}
 
 
@@ -103,6 +103,7 @@ _SIGS = {
     "dsa_chain_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "dsa_chain_set_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
     "dsa_group_matrices": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "dsa_group_field": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "dsa_cfg_save": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64]),
     "dsa_cfg_load": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64]),
     "dsa_chain_save_cfg": (C.c_int, [C.c_void_p, C.c_char_p]),
@@ -182,6 +183,17 @@ class Group:
         _check(lib().dsa_group_matrices(self._h, _ptr(raw)))
         m = raw.reshape(self.P, 3, 3, 2)
         return m[..., 0] + 1j * m[..., 1]
+
+    def field(self):
+        """The group's GF(4) form: (mat, key), mat[e] the 3x3 matrix of e over
+        GF(4) as a uint8 [P, 3, 3] array of entries 2 e1 + e0 (for e1 w + e0),
+        key the element of each first-two-rows code (4096 uint16, 0xFFFF where
+        none); raises if the group has none."""
+        mat = np.empty(self.P, np.uint32)
+        key = np.empty(4096, np.uint16)
+        _check(lib().dsa_group_field(self._h, _ptr(mat), _ptr(key)))
+        ent = (mat[:, None] >> (2 * np.arange(9, dtype=np.uint32))) & 3
+        return ent.astype(np.uint8).reshape(self.P, 3, 3), key
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -269,13 +281,15 @@ class Chain:
         values in LDS, exact re-decision near the bound), "class-narrow" (the
         byte class-index form), "matrix" (trial actions as FP32 traces of the group's SU(3)
         matrices in LDS, exact re-decision near the rounding bound; groups
-        whose file carries the matrices), or "auto"; all give identical
-        links.  A huge margin_scale re-decides every hit exactly."""
+        whose file carries the matrices), "field" (the byte class-index form
+        with the staple products as GF(4) matrix products; groups with a
+        GF(4) form), or "auto"; all give identical links.  A huge
+        margin_scale re-decides every hit exactly."""
         _check(lib().dsa_chain_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
 
     def kernel_form(self) -> str:
         """The kernel the chain's passes run: "table", "class" (the int16-value
-        form), "class-narrow" or "matrix"."""
+        form), "class-narrow", "matrix" or "field"."""
         f = C.c_int32()
         _check(lib().dsa_chain_kernel_form(self._h, C.byref(f)))
         return {v: k for k, v in DSA_KERNEL.items()}[f.value]

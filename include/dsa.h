@@ -70,6 +70,14 @@ int dsa_group_tables(const dsa_group *g, int32_t *mult, int32_t *inv, double *re
  * its table, as dym-mod-metro-savecfg.cpp:249-262 reads them: out[18*e + part
  * + 2*(c + 3*r)], P*18 doubles.  DSA_E_STATE if the file has none. */
 int dsa_group_matrices(const dsa_group *g, double *out);
+/* No counterpart in the reference (the sweep kernels' product form): the
+ * group's representation by 3x3 matrices over GF(4) = {0, 1, w, w^2}, found
+ * at load time and checked against every product of the table.  mat[e] holds
+ * element e's matrix with entry (r, c) in bits 2(3r + c) (2 e1 + e0 for
+ * e1 w + e0), P uint32; either pointer may be NULL; key (4096 uint16) maps
+ * the first two rows of a matrix (mat & 0xFFF) to its element, 0xFFFF where
+ * none.  DSA_E_STATE if the group has no such form. */
+int dsa_group_field(const dsa_group *g, uint32_t *mat, uint16_t *key);
 void dsa_group_free(dsa_group *g);
 
 /* Replaces main's set-up (dym-mod-metro.cpp:222-268): volume, per-site
@@ -166,14 +174,20 @@ int dsa_chain_save_kentucky(dsa_chain *c, const char *prefix, int32_t num, char 
  * 3x3 complex matrix from those matrices in LDS, and each trial action as one
  * trace of a product with it; a hit whose decision the rounding bound could
  * change is decided again from element values.
+ * FIELD (groups with a GF(4) form: a faithful representation by 3x3
+ * matrices over GF(4), found and checked against every product of the table
+ * when the group is loaded, e.g. S(1080), S(216), S(108)): the byte-index
+ * class update with its staple-times-link products formed as packed GF(4)
+ * matrix products in registers instead of table gathers; AUTO takes it
+ * wherever CLASS_NARROW would run and the form exists.
  * margin_scale (default 1) multiplies the bound
  * that selects those re-decisions: a huge value makes every hit re-decided
  * (a test of that path). */
 enum { DSA_KERNEL_AUTO = 0, DSA_KERNEL_TABLE = 1, DSA_KERNEL_CLASS = 2, DSA_KERNEL_CLASS_NARROW = 3,
-       DSA_KERNEL_MATRIX = 4 };
+       DSA_KERNEL_MATRIX = 4, DSA_KERNEL_FIELD = 5 };
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale);
 /* The kernel the chain's passes run: TABLE, CLASS (the int16-value form),
- * CLASS_NARROW (the byte-index form) or MATRIX. */
+ * CLASS_NARROW (the byte-index form), MATRIX or FIELD. */
 int dsa_chain_kernel_form(dsa_chain *c, int32_t *form);
 /* Hits the CLASS or MATRIX kernel re-decided from element values since creation. */
 int dsa_chain_exact_decisions(dsa_chain *c, uint64_t *n);

@@ -122,18 +122,27 @@ def test_cli_replay_reproduces_reference_output():
     ("mys1080-v4", 4, 4, 6, 0.0, 4.0, -0.3, 16, "cold", 3),
 ])
 @pytest.mark.parametrize("rng", ["philox", "minstd"])
-@pytest.mark.parametrize("kernel", ["table", "class", "class-narrow", "class-all-exact"])
+@pytest.mark.parametrize("kernel", ["table", "class", "class-narrow", "class-all-exact", "field", "field-all-exact"])
 def test_colour_order_matches_oracle(dsa, dgroups, oracle, ogroups, case, rng, kernel):
-    """Every sweep kernel against the oracle; class-all-exact widens the
-    class kernel's bound so that every hit takes its exact re-decision path."""
+    """Every sweep kernel against the oracle; class-all-exact and
+    field-all-exact widen the class kernel's bound so that every hit takes its
+    exact re-decision path.  The field kernel (GF(4) products) runs every
+    pass here, small ones included; groups with no GF(4) form (myBI, myBT)
+    refuse it."""
     name, D, Nt, Nx, b0, b1, b2, seed, start, nsw = case
     say("case", name, D, Nt, Nx, rng, kernel)
     o = oracle.OChain(ogroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour")
     with dsa.Chain(dgroups(name), D, Nt, Nx, b0, b1, b2, seed, start=start, rng=rng, order="colour") as c:
-        if kernel == "class-all-exact":
-            c.set_kernel("class", 1e300)
+        if kernel.startswith("field") and name in ("myBI", "myBT"):
+            with pytest.raises(dsa.DSAError):
+                c.set_kernel("field")
+            return
+        if kernel.endswith("-all-exact"):
+            c.set_kernel(kernel[: -len("-all-exact")], 1e300)
         else:
             c.set_kernel(kernel)
+        if kernel.startswith("field"):
+            assert c.kernel_form() == "field"
         assert np.array_equal(c.a, o.a), "start differs"
         for s in range(nsw):
             c.update(1)
@@ -142,7 +151,7 @@ def test_colour_order_matches_oracle(dsa, dgroups, oracle, ogroups, case, rng, k
         assert c.counters() == o.counters()
         if rng == "minstd":
             assert np.array_equal(c.rng_state(), o.mstate())
-        if kernel == "class-all-exact":
+        if kernel.endswith("-all-exact"):
             assert c.exact_decisions() == c.counters()[0]
         m = c.measure(min(Nx, 3), min(Nt, 3), sum="ordered")
         assert np.array_equal(m, o.measure(min(Nx, 3), min(Nt, 3), mode=0))
@@ -211,8 +220,9 @@ def test_config2_shape_one_sweep_matches_oracle(dsa, dgroups, oracle, ogroups):
     o = oracle.OChain(ogroups("mys1080-v4"), 4, 32, 16, 0.0, 3.0, -0.25, 7, rng="philox", order="colour")
     with dsa.Chain(dgroups("mys1080-v4"), 4, 32, 16, 0.0, 3.0, -0.25, 7) as c:
         assert np.array_equal(c.a, o.a)
-        # the default choice takes the byte-index class kernel at this size
-        assert c.kernel_form() == "class-narrow"
+        # the default choice: the byte-index class update, its large passes
+        # with GF(4) products (this size's passes take the small-pass kernel)
+        assert c.kernel_form() == "field"
         c.update(1)
         o.sweep(1)
         assert np.array_equal(c.a, o.a)
@@ -238,13 +248,14 @@ def test_mid_size_passes_class_equals_table(dsa, dgroups, Nt, Nx):
     say("case class vs table", Nt, Nx)
     g = dgroups("mys1080-v4")
     out = []
-    for kernel in ("table", "class-narrow"):
+    for kernel in ("table", "class-narrow", "field", "auto"):
         with dsa.Chain(g, 4, Nt, Nx, 0.0, 3.0, -0.25, 21) as c:
             c.set_kernel(kernel)
             c.update(2)
             out.append((c.a, c.counters()))
-    assert np.array_equal(out[0][0], out[1][0])
-    assert out[0][1] == out[1][1]
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0])
+        assert out[0][1] == o[1]
 
 
 def test_profiling_modes_count_launches(dsa, dgroups):
