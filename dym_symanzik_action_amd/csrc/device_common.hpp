@@ -26,6 +26,15 @@ __device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
   return (__attribute__((address_space(1))) T *)p;
 }
 
+// The uint16 at byte offset `off` of a global array: the load takes the
+// base from scalar registers and one vector register of offset, with no
+// 64-bit address arithmetic per load.
+__device__ inline uint32_t u16_at(const uint16_t *base, uint32_t off) {
+  using gchar = const __attribute__((address_space(1))) unsigned char;
+  using gu16 = const __attribute__((address_space(1))) uint16_t;
+  return *reinterpret_cast<gu16 *>(reinterpret_cast<gchar *>(gmem(base)) + off);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD one
 // contiguous range of the pass so neighbouring links share an L2.
 __device__ inline uint32_t xcd_block() {
@@ -101,20 +110,27 @@ __device__ inline uint32_t lpos(const Geom &G, uint32_t site, uint32_t par) {
   return G.eo ? (par ? G.halfV : 0u) + (site >> 1) : site;
 }
 
-// Link U_dir(x + a e_mu + b e_nu).
+// Link U_dir(x + a e_mu + b e_nu).  A chain has fewer than 2^31 links, so
+// the byte offset fits 32 bits.
 template <int D>
 __device__ inline uint32_t U(const uint16_t *links, const Geom &G, const Site<D> &s, int dir, int mu,
                              int a, int nu, int b) {
   uint32_t par;
   const uint32_t site = shifted<D>(G, s, mu, a, nu, b, &par);
-  return gmem(links)[(size_t)dir * G.V + lpos(G, site, par)];
+  return u16_at(links, 2u * ((uint32_t)dir * G.V + lpos(G, site, par)));
+}
+
+// mult[a * P + b]: the order is at most 46340, so the byte offset a 2P + 2b
+// fits 32 bits (and a < 2^24, 2P < 2^24 for the 24-bit multiply).
+__device__ inline uint32_t table_at(const uint16_t *mult, uint32_t P, uint32_t a, uint32_t b) {
+  return u16_at(mult, __umul24(a, 2u * P) + 2u * b);
 }
 
 struct TabView {
   const uint16_t *__restrict__ mult;
   const uint16_t *inv;  // LDS or global
   uint32_t P;
-  __device__ inline uint32_t M(uint32_t a, uint32_t b) const { return gmem(mult)[a * P + b]; }
+  __device__ inline uint32_t M(uint32_t a, uint32_t b) const { return table_at(mult, P, a, b); }
   __device__ inline uint32_t I(uint32_t a) const { return inv[a]; }
 };
 
@@ -461,10 +477,14 @@ struct ClsNarrow {
   static constexpr double kScale = 0x1p-24;
   static constexpr bool kBound = true;
   using Row = const uint8_t *;
-  __device__ Row row(uint32_t jp, uint32_t P) const { return s_cprod + jp * P; }
+  __device__ Row row(uint32_t jp, uint32_t P) const { return s_cprod + __umul24(jp, P); }  // both < 2^24
   __device__ Row row0() const { return s_cls; }
-  __device__ int32_t val(Row r, uint32_t t) const {
-    return *reinterpret_cast<const int32_t *>(reinterpret_cast<const uint8_t *>(s_rcq) + r[t]);
+  static constexpr bool kTwoLevel = true;
+  __device__ int32_t val(Row r, uint32_t t) const { return at(idx(r, t)); }
+  // the two levels of val: the class byte, then its value
+  __device__ uint32_t idx(Row r, uint32_t t) const { return r[t]; }
+  __device__ int32_t at(uint32_t off) const {
+    return *reinterpret_cast<const int32_t *>(reinterpret_cast<const uint8_t *>(s_rcq) + off);
   }
   // the class value of t and its bound, from one class-index read
   __device__ int32_t val_md(uint32_t t, int32_t &md) const {
@@ -479,8 +499,9 @@ struct ClsWide {
   const int16_t *s_cval0;  // LDS, P: class value of t
   static constexpr double kScale = 0x1p-13;
   static constexpr bool kBound = false;
+  static constexpr bool kTwoLevel = false;
   using Row = const int16_t *;
-  __device__ Row row(uint32_t jp, uint32_t P) const { return s_cval + jp * P; }
+  __device__ Row row(uint32_t jp, uint32_t P) const { return s_cval + __umul24(jp, P); }
   __device__ Row row0() const { return s_cval0; }
   __device__ int32_t val(Row r, uint32_t t) const { return r[t]; }
 };
@@ -518,10 +539,23 @@ template <int D, class L>
 __device__ inline void cls_sums(const L &Lk, typename L::Row row, const uint32_t *t, int32_t &r, int32_t &m) {
   constexpr int NS = 2 * (D - 1), NL = 8 * (D - 1);
   int32_t a = 0, c = 0;
+  if constexpr (L::kTwoLevel) {
+    // every class byte first, then the values: two LDS round trips per hit
+    // instead of a chain of short ones
+    uint32_t off[NL];
 #pragma unroll
-  for (int k = 0; k < NL; ++k) {
-    const int32_t v = Lk.val(row, t[k]);
-    if (k < NS) a += v; else c += v;
+    for (int k = 0; k < NL; ++k) off[k] = Lk.idx(row, t[k]);
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int32_t v = Lk.at(off[k]);
+      if (k < NS) a += v; else c += v;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int32_t v = Lk.val(row, t[k]);
+      if (k < NS) a += v; else c += v;
+    }
   }
   r = a;
   m = c;

@@ -42,7 +42,7 @@ std::string load_group(const char *path, Group &g) {
   char *e = nullptr;
   long Pl = strtol(p, &e, 10);
   if (e == p) return "group file: missing order";
-  if (Pl <= 0 || Pl > (long)kMaxOrder) return "group file: order " + std::to_string(Pl) + " outside [1, 65535]";
+  if (Pl <= 0 || Pl > (long)kMaxOrder) return "group file: order " + std::to_string(Pl) + " outside [1, " + std::to_string(kMaxOrder) + "]";
   p = e;
   const unsigned P = (unsigned)Pl;
   g.P = P;

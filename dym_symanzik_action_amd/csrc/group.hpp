@@ -7,8 +7,9 @@
 
 namespace dsa {
 
-// Largest order whose element indices fit the uint16 link encoding.
-constexpr unsigned kMaxOrder = 65535;
+// Largest order whose P x P uint16 table the kernels address with 32-bit
+// byte offsets (P^2 * 2 < 2^32); element indices then fit the uint16 links.
+constexpr unsigned kMaxOrder = 46340;
 
 struct Group {
   std::string path;
