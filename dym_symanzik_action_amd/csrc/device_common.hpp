@@ -26,6 +26,9 @@ __device__ inline __attribute__((address_space(1))) T *gmem(T *p) {
   return (__attribute__((address_space(1))) T *)p;
 }
 
+// Four dwords as one native vector: 16-byte LDS and global accesses.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // The uint16 at byte offset `off` of a global array: the load takes the
 // base from scalar registers and one vector register of offset, with no
 // 64-bit address arithmetic per load.
@@ -252,68 +255,93 @@ __device__ inline void build_t(const uint16_t *links, const Geom &G, const TabVi
 }
 
 // Group products from the GF(4) form (gf4.hpp) instead of the table: the
-// packed matrix of an element and of its inverse, and the element of a
-// packed matrix, all in LDS.
+// packed matrix of each element and of its inverse in LDS (8 bytes per
+// element), split into product factors in registers where used; an element
+// is found from its matrix through the key table.
 struct FieldView {
   const uint32_t *m;    // P
   const uint32_t *mi;   // P: matrix of the inverse
   const uint16_t *key;  // kGf4Keys
-  __device__ inline uint32_t M(uint32_t e) const { return m[e]; }
-  __device__ inline uint32_t Mi(uint32_t e) const { return mi[e]; }
+  __device__ inline Gf4L L(uint32_t e) const { return gf4_left(m[e]); }
+  __device__ inline Gf4L Li(uint32_t e) const { return gf4_left(mi[e]); }
+  __device__ inline Gf4R R(uint32_t e) const { return gf4_right(m[e]); }
+  __device__ inline Gf4R Ri(uint32_t e) const { return gf4_right(mi[e]); }
   __device__ inline uint32_t E(uint32_t w) const { return key[gf4_key(w)]; }
 };
 
 // build_t with every product a packed GF(4) matrix product (no memory
-// access; the same 85 products): the links' matrices and their inverses'
-// come from LDS, and each t_k's element from its key.
+// access; the same 85 products).  All 62 links are loaded before the first
+// product, so a wavefront waits on global memory once per link-update.
+// Each link's matrix (or its inverse's) is then read from LDS and split
+// into the factor its one role needs (left or right); shared products are
+// split once.
 template <int D>
 __device__ inline void build_t_field(const uint16_t *links, const Geom &G, const FieldView &F, const Site<D> &s,
                                      int mu, uint32_t b, uint32_t *t) {
   constexpr int NS = 2 * (D - 1);
-  const uint32_t Mb = F.M(b);
-  const uint32_t umu_p = F.M(U<D>(links, G, s, mu, mu, 1, mu, 0));             // U_mu(x+mu)
-  const uint32_t C = gf4_mul3(F.M(U<D>(links, G, s, mu, mu, -1, mu, 0)), Mb);  // U_mu(x-mu) b
+  // the links of each transverse direction nu, in the order used below
+  uint32_t e[D - 1][20];
+  const uint32_t e_mp = U<D>(links, G, s, mu, mu, 1, mu, 0);   // U_mu(x+mu)
+  const uint32_t e_mm = U<D>(links, G, s, mu, mu, -1, mu, 0);  // U_mu(x-mu)
 #pragma unroll
   for (int j = 0; j < D - 1; ++j) {
     const int nu = j + (j >= mu ? 1 : 0);
-    const uint32_t n_10 = F.M(U<D>(links, G, s, nu, mu, 1, nu, 0));
-    const uint32_t n_00i = F.Mi(U<D>(links, G, s, nu, mu, 0, nu, 0));
-    const uint32_t n_1mi = F.Mi(U<D>(links, G, s, nu, mu, 1, nu, -1));
-    const uint32_t n_0m = F.M(U<D>(links, G, s, nu, mu, 0, nu, -1));
-    const uint32_t n_11 = F.M(U<D>(links, G, s, nu, mu, 1, nu, 1));
-    const uint32_t n_01i = F.Mi(U<D>(links, G, s, nu, mu, 0, nu, 1));
-    const uint32_t n_20 = F.M(U<D>(links, G, s, nu, mu, 2, nu, 0));
-    const uint32_t n_m0i = F.Mi(U<D>(links, G, s, nu, mu, -1, nu, 0));
-    const uint32_t n_1Mi = F.Mi(U<D>(links, G, s, nu, mu, 1, nu, -2));
-    const uint32_t n_0M = F.M(U<D>(links, G, s, nu, mu, 0, nu, -2));
-    const uint32_t n_2mi = F.Mi(U<D>(links, G, s, nu, mu, 2, nu, -1));
-    const uint32_t n_mm = F.M(U<D>(links, G, s, nu, mu, -1, nu, -1));
-    const uint32_t m_01i = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, 1));
-    const uint32_t m_0mi = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, -1));
-    const uint32_t m_02i = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, 2));
-    const uint32_t m_11i = F.Mi(U<D>(links, G, s, mu, mu, 1, nu, 1));
-    const uint32_t m_m1i = F.Mi(U<D>(links, G, s, mu, mu, -1, nu, 1));
-    const uint32_t m_0Mi = F.Mi(U<D>(links, G, s, mu, mu, 0, nu, -2));
-    const uint32_t m_1mi = F.Mi(U<D>(links, G, s, mu, mu, 1, nu, -1));
-    const uint32_t m_mmi = F.Mi(U<D>(links, G, s, mu, mu, -1, nu, -1));
-    const uint32_t A = gf4_mul3(n_00i, Mb);
-    const uint32_t Bd = gf4_mul3(n_0m, Mb);
-    const uint32_t X = gf4_mul3(n_10, m_01i);
-    const uint32_t Y = gf4_mul3(n_1mi, m_0mi);
-    const uint32_t p0 = gf4_mul3(n_10, n_11), q0 = gf4_mul3(m_02i, n_01i);
-    const uint32_t p1 = gf4_mul3(umu_p, n_20), q1 = gf4_mul3(m_11i, m_01i);
-    const uint32_t q2 = gf4_mul3(m_m1i, n_m0i);
-    const uint32_t p3 = gf4_mul3(n_1mi, n_1Mi), q3 = gf4_mul3(m_0Mi, n_0M);
-    const uint32_t p4 = gf4_mul3(umu_p, n_2mi), q4 = gf4_mul3(m_1mi, m_0mi);
-    const uint32_t q5 = gf4_mul3(m_mmi, n_mm);
-    t[2 * j] = F.E(gf4_mul3(X, A));
-    t[2 * j + 1] = F.E(gf4_mul3(Y, Bd));
-    t[NS + 6 * j + 0] = F.E(gf4_mul3(gf4_mul3(p0, q0), A));
-    t[NS + 6 * j + 1] = F.E(gf4_mul3(gf4_mul3(p1, q1), A));
-    t[NS + 6 * j + 2] = F.E(gf4_mul3(gf4_mul3(X, q2), C));
-    t[NS + 6 * j + 3] = F.E(gf4_mul3(gf4_mul3(p3, q3), Bd));
-    t[NS + 6 * j + 4] = F.E(gf4_mul3(gf4_mul3(p4, q4), Bd));
-    t[NS + 6 * j + 5] = F.E(gf4_mul3(gf4_mul3(Y, q5), C));
+    e[j][0] = U<D>(links, G, s, nu, mu, 1, nu, 0);
+    e[j][1] = U<D>(links, G, s, nu, mu, 0, nu, 0);
+    e[j][2] = U<D>(links, G, s, nu, mu, 1, nu, -1);
+    e[j][3] = U<D>(links, G, s, nu, mu, 0, nu, -1);
+    e[j][4] = U<D>(links, G, s, nu, mu, 1, nu, 1);
+    e[j][5] = U<D>(links, G, s, nu, mu, 0, nu, 1);
+    e[j][6] = U<D>(links, G, s, nu, mu, 2, nu, 0);
+    e[j][7] = U<D>(links, G, s, nu, mu, -1, nu, 0);
+    e[j][8] = U<D>(links, G, s, nu, mu, 1, nu, -2);
+    e[j][9] = U<D>(links, G, s, nu, mu, 0, nu, -2);
+    e[j][10] = U<D>(links, G, s, nu, mu, 2, nu, -1);
+    e[j][11] = U<D>(links, G, s, nu, mu, -1, nu, -1);
+    e[j][12] = U<D>(links, G, s, mu, mu, 0, nu, 1);
+    e[j][13] = U<D>(links, G, s, mu, mu, 0, nu, -1);
+    e[j][14] = U<D>(links, G, s, mu, mu, 0, nu, 2);
+    e[j][15] = U<D>(links, G, s, mu, mu, 1, nu, 1);
+    e[j][16] = U<D>(links, G, s, mu, mu, -1, nu, 1);
+    e[j][17] = U<D>(links, G, s, mu, mu, 0, nu, -2);
+    e[j][18] = U<D>(links, G, s, mu, mu, 1, nu, -1);
+    e[j][19] = U<D>(links, G, s, mu, mu, -1, nu, -1);
+  }
+  // keep every load above this point: the scheduler would otherwise sink
+  // each load to its first use and wait on memory once per batch
+  __builtin_amdgcn_sched_barrier(0);
+  const Gf4R Rb = F.R(b);
+  const Gf4L umu_p = F.L(e_mp);
+  const Gf4R C = gf4_right(gf4_mul(F.L(e_mm), Rb));  // U_mu(x-mu) b
+#pragma unroll
+  for (int j = 0; j < D - 1; ++j) {
+    const Gf4L n_10 = F.L(e[j][0]), n_00i = F.Li(e[j][1]), n_1mi = F.Li(e[j][2]), n_0m = F.L(e[j][3]);
+    const Gf4R n_11 = F.R(e[j][4]), n_01i = F.Ri(e[j][5]), n_20 = F.R(e[j][6]), n_m0i = F.Ri(e[j][7]);
+    const Gf4R n_1Mi = F.Ri(e[j][8]), n_0M = F.R(e[j][9]), n_2mi = F.Ri(e[j][10]), n_mm = F.R(e[j][11]);
+    const Gf4R m_01i = F.Ri(e[j][12]), m_0mi = F.Ri(e[j][13]);
+    const Gf4L m_02i = F.Li(e[j][14]), m_11i = F.Li(e[j][15]), m_m1i = F.Li(e[j][16]), m_0Mi = F.Li(e[j][17]);
+    const Gf4L m_1mi = F.Li(e[j][18]), m_mmi = F.Li(e[j][19]);
+    // level 1: products of two links (A, Bd with b), right or left forms
+    // as their later products need them
+    const Gf4R A = gf4_right(gf4_mul(n_00i, Rb));         // U_nu(x)^-1 b
+    const Gf4R Bd = gf4_right(gf4_mul(n_0m, Rb));         // U_nu(x-nu) b
+    const uint32_t X = gf4_mul(n_10, m_01i);              // plaquette-up prefix
+    const uint32_t Y = gf4_mul(n_1mi, m_0mi);             // plaquette-down prefix
+    const Gf4L XL = gf4_left(X), YL = gf4_left(Y);
+    const Gf4L p0 = gf4_left(gf4_mul(n_10, n_11)), p1 = gf4_left(gf4_mul(umu_p, n_20));
+    const Gf4L p3 = gf4_left(gf4_mul(n_1mi, n_1Mi)), p4 = gf4_left(gf4_mul(umu_p, n_2mi));
+    const Gf4R q0 = gf4_right(gf4_mul(m_02i, n_01i)), q1 = gf4_right(gf4_mul(m_11i, m_01i));
+    const Gf4R q2 = gf4_right(gf4_mul(m_m1i, n_m0i)), q3 = gf4_right(gf4_mul(m_0Mi, n_0M));
+    const Gf4R q4 = gf4_right(gf4_mul(m_1mi, m_0mi)), q5 = gf4_right(gf4_mul(m_mmi, n_mm));
+    // levels 2 and 3
+    t[2 * j] = F.E(gf4_mul(XL, A));
+    t[2 * j + 1] = F.E(gf4_mul(YL, Bd));
+    t[NS + 6 * j + 0] = F.E(gf4_mul(gf4_left(gf4_mul(p0, q0)), A));
+    t[NS + 6 * j + 1] = F.E(gf4_mul(gf4_left(gf4_mul(p1, q1)), A));
+    t[NS + 6 * j + 2] = F.E(gf4_mul(gf4_left(gf4_mul(XL, q2)), C));
+    t[NS + 6 * j + 3] = F.E(gf4_mul(gf4_left(gf4_mul(p3, q3)), Bd));
+    t[NS + 6 * j + 4] = F.E(gf4_mul(gf4_left(gf4_mul(p4, q4)), Bd));
+    t[NS + 6 * j + 5] = F.E(gf4_mul(gf4_left(gf4_mul(YL, q5)), C));
   }
 }
 
@@ -892,7 +920,6 @@ __device__ inline uint32_t update_link_mat(const LinkCtx &C, const MatCtx &K, co
 // iteration, ~18 for the S(1080) class tables at 512 lanes).
 // 16-byte unit of the staged copies: a native vector type, which the
 // compiler keeps in registers (arrays of HIP's uint4 struct land in scratch).
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <int K, typename T>
 struct Staged {

@@ -23,26 +23,54 @@ constexpr uint32_t kGf4Col = 0x49u * (1u | 1u << 10 | 1u << 20);  // entries (r,
 constexpr uint32_t kGf4Row = 0x7u * (1u | 1u << 10 | 1u << 20);   // entries (0, c) of each plane
 constexpr uint32_t kGf4Keys = 4096;                                 // first two rows: 6 entries x 2 bits
 
+// A[r][k] across row r of every plane (the k-th term's left factor).
+DSA_HD inline uint32_t gf4_col(uint32_t A, int k) {
+  uint32_t a = (A >> k) & kGf4Col;  // A[r][k] at (r, 0)
+#if defined(__HIP_DEVICE_COMPILE__)
+  // hide a's known-zero bits: the compiler would otherwise turn the
+  // replication into a * 7, a quarter-rate 32-bit multiply (a reaches bit
+  // 26, past the full-rate 24-bit one); two shift-ors are full rate
+  asm("" : "+v"(a));
+#endif
+  return a | (a << 1) | (a << 2);
+}
+
+// B[k][c] down column c of every plane (the k-th term's right factor).
+DSA_HD inline uint32_t gf4_row(uint32_t B, int k) {
+  const uint32_t b = (B >> (3 * k)) & kGf4Row;  // B[k][c] at (0, c)
+  return b | (b << 3) | (b << 6);
+}
+
+// The product from the sum S of its three outer-product terms.  The planes
+// of S are f0 = Al Bl, f1 = (Al + Ah)(Bl + Bh), f2 = Ah Bh; the product's lo
+// plane is f0 + f2, its hi plane f0 + f1, and lo ^ hi = f1 + f2.
+DSA_HD inline uint32_t gf4_fin(uint32_t S) {
+  const uint32_t u = S ^ (S >> 10), v = S ^ (S >> 20);
+  return (v & 0x1FFu) | (u & (0x1FFu << 10)) | (u << 20);
+}
+
 DSA_HD inline uint32_t gf4_mul3(uint32_t A, uint32_t B) {
   uint32_t S = 0;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    uint32_t a = (A >> k) & kGf4Col;  // A[r][k] at (r, 0)
-#if defined(__HIP_DEVICE_COMPILE__)
-    // hide a's known-zero bits: the compiler would otherwise turn the
-    // replication into a * 7, a quarter-rate 32-bit multiply (a reaches bit
-    // 26, past the full-rate 24-bit one); two shift-ors are full rate
-    asm("" : "+v"(a));
-#endif
-    a |= (a << 1) | (a << 2);         // ... across row r
-    uint32_t b = (B >> (3 * k)) & kGf4Row;  // B[k][c] at (0, c)
-    b |= (b << 3) | (b << 6);               // ... down column c
-    S ^= a & b;
-  }
-  // planes of S: f0 = Al Bl, f1 = (Al + Ah)(Bl + Bh), f2 = Ah Bh; the
-  // product's lo plane is f0 + f2, its hi plane f0 + f1, and lo ^ hi = f1 + f2
-  const uint32_t u = S ^ (S >> 10), v = S ^ (S >> 20);
-  return (v & 0x1FFu) | (u & (0x1FFu << 10)) | (u << 20);
+  for (int k = 0; k < 3; ++k) S ^= gf4_col(A, k) & gf4_row(B, k);
+  return gf4_fin(S);
+}
+
+// An operand with its three term factors formed ahead: as the left factor
+// of products (columns) or as the right one (rows).  The sweep kernels read
+// the links' matrices in these forms from tables.
+struct Gf4L {
+  uint32_t c[3];
+};
+struct Gf4R {
+  uint32_t r[3];
+};
+
+DSA_HD inline Gf4L gf4_left(uint32_t A) { return Gf4L{{gf4_col(A, 0), gf4_col(A, 1), gf4_col(A, 2)}}; }
+DSA_HD inline Gf4R gf4_right(uint32_t B) { return Gf4R{{gf4_row(B, 0), gf4_row(B, 1), gf4_row(B, 2)}}; }
+
+DSA_HD inline uint32_t gf4_mul(const Gf4L &a, const Gf4R &b) {
+  return gf4_fin((a.c[0] & b.r[0]) ^ (a.c[1] & b.r[1]) ^ (a.c[2] & b.r[2]));
 }
 
 // The first two rows of the packed matrix (lo and hi planes) as a 12-bit key.
