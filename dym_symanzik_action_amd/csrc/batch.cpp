@@ -35,6 +35,7 @@ struct HostChain {
   dsa::DeviceTables dt;
   size_t links_off = 0, sites_off = 0, ms_off = 0;
   bool cls = false;     // class-product update
+  bool field = false;   // ... with GF(4) products
   double margin = 0;    // its decision bound at margin scale 1
 };
 
@@ -255,7 +256,14 @@ int dsa_batch_finalize(dsa_batch *b) {
              (b->kernel != DSA_KERNEL_AUTO || hc.margin < 1e-3);
     if (b->kernel != DSA_KERNEL_TABLE && b->kernel != DSA_KERNEL_AUTO && !hc.cls)
       return fail(DSA_E_STATE, "no class tables for a chain's group on this device");
-    b->lds = std::max(b->lds, hc.cls ? dsa::sweep_cls_lds_bytes(T, dsa::kClsNarrow) : dsa::sweep_lds_bytes(T));
+    // with the GF(4) products where the group has that form and it fits
+    hc.field = hc.cls && T.fmat && (b->kernel == DSA_KERNEL_AUTO || b->kernel == DSA_KERNEL_FIELD) &&
+               dsa::sweep_cls_lds_bytes(T, dsa::kClsField) + 256 <= (size_t)lds_max;
+    if (b->kernel == DSA_KERNEL_FIELD && !hc.field)
+      return fail(DSA_E_STATE, "no GF(4) form of a chain's group in this device's LDS");
+    b->lds = std::max(b->lds, hc.field ? dsa::sweep_cls_lds_bytes(T, dsa::kClsField)
+                              : hc.cls ? dsa::sweep_cls_lds_bytes(T, dsa::kClsNarrow)
+                                       : dsa::sweep_lds_bytes(T));
     std::string err = hc.p.order == DSA_ORDER_COLOUR ? dsa::build_colour_schedule(hc.G, hc.S)
                                                      : dsa::build_reference_schedule(hc.G, hc.S);
     if (!err.empty()) return fail(DSA_E_STATE, err);
@@ -302,6 +310,7 @@ int dsa_batch_finalize(dsa_batch *b) {
     d.rng = hc.p.rng == DSA_RNG_MINSTD ? dsa::kRngMinstd : dsa::kRngPhilox;
     d.sweep_base = 0;
     d.cls = hc.cls ? 1 : 0;
+    d.field = hc.field ? 1 : 0;
     d.cmargin = hc.margin * b->margin_scale;
     d.fall = b->d_fall;
     d.bound = 0;
@@ -391,7 +400,7 @@ int dsa_batch_set_kernel(dsa_batch *b, int32_t kernel, double margin_scale) {
   if (!b) return fail(DSA_E_ARG, "dsa_batch_set_kernel: NULL batch");
   if (b->final) return fail(DSA_E_STATE, "dsa_batch_set_kernel after dsa_batch_finalize");
   if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS &&
-      kernel != DSA_KERNEL_CLASS_NARROW)
+      kernel != DSA_KERNEL_CLASS_NARROW && kernel != DSA_KERNEL_FIELD)
     return fail(DSA_E_ARG, "dsa_batch_set_kernel: unknown kernel " + std::to_string(kernel));
   if (!(margin_scale >= 1.0) || !std::isfinite(margin_scale))
     return fail(DSA_E_ARG, "dsa_batch_set_kernel: margin_scale must be finite and >= 1");

@@ -195,7 +195,9 @@ constexpr size_t cls_small_static_lds(int nl) {
   return 64 * sizeof(int32_t) + (size_t)kNHit * nl * (sizeof(double) + sizeof(uint16_t));
 }
 
-template <int D, int NL, bool B>
+// F: the staple-times-link products from the group's GF(4) form, its
+// tables after the narrow ones (as in k_sweep_cls).
+template <int D, int NL, bool B, bool F = false>
 __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t s_rcq[64];
@@ -207,7 +209,24 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
   unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
   unsigned char *s_tab = s_tab0 + pad16(P);
   int32_t *s_md = A.T.mdq ? reinterpret_cast<int32_t *>(s_tab + pad16((size_t)np * P)) : nullptr;
-  {
+  unsigned char *s_f = s_tab + pad16((size_t)np * P) + (A.T.mdq ? pad16(4 * (size_t)A.T.ncls) : 0);
+  uint32_t *s_fm = reinterpret_cast<uint32_t *>(s_f);
+  uint32_t *s_fmi = reinterpret_cast<uint32_t *>(s_f + pad16(4 * (size_t)P));
+  uint16_t *s_fkey = reinterpret_cast<uint16_t *>(s_f + 2 * pad16(4 * (size_t)P));
+  if constexpr (F) {
+    constexpr uint32_t nk = kGf4Keys * sizeof(uint16_t) / 16;
+    const u32x4 *ks = reinterpret_cast<const u32x4 *>(A.T.fkey);
+    Staged<3, uint32_t> fm, fmi;
+    Staged<1, u32x4> fk;
+    fm.load(A.T.fmat, P);
+    fmi.load(A.T.finv, P);
+    fk.load(ks, nk);
+    ClsNarrowStage g;
+    g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
+    fm.store(s_fm, A.T.fmat, P);
+    fmi.store(s_fmi, A.T.finv, P);
+    fk.store(reinterpret_cast<u32x4 *>(s_fkey), ks, nk);
+  } else {
     ClsNarrowStage g;
     g.run(A.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
   }
@@ -234,7 +253,14 @@ __global__ __launch_bounds__(kClsBlockNarrow) void k_sweep_cls_small(SweepArgs A
   uint32_t nfall = 0;
   if (threadIdx.x < n) {
     const PreDraws pre{s_jp, s_u2, (uint32_t)NL, threadIdx.x};
-    acc = update_link_cls<D, B>(C, Lk, K, A.G, T, A.mu, A.sites[first + threadIdx.x], nfall, &pre);
+    if constexpr (F) {
+      const FieldView FV{s_fm, s_fmi, s_fkey};
+      ClsLink<D> S;
+      cls_prepare_field<D, B>(S, C, Lk, A.G, FV, A.mu, A.sites[first + threadIdx.x]);
+      acc = cls_hits<D, B>(S, C, Lk, K, T, A.mu, nfall, &pre);
+    } else {
+      acc = update_link_cls<D, B>(C, Lk, K, A.G, T, A.mu, A.sites[first + threadIdx.x], nfall, &pre);
+    }
   }
   acc = wave_sum_u64(acc);
   const unsigned long long nf = wave_sum_u64(nfall);
@@ -265,7 +291,24 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     unsigned char *s_tab0 = reinterpret_cast<unsigned char *>(s_pool) + pad16(2 * (size_t)np);
     unsigned char *s_tab = s_tab0 + pad16(P);
     int32_t *s_md = BC.T.mdq ? reinterpret_cast<int32_t *>(s_tab + pad16((size_t)np * P)) : nullptr;
-    {
+    unsigned char *s_f = s_tab + pad16((size_t)np * P) + (BC.T.mdq ? pad16(4 * (size_t)BC.T.ncls) : 0);
+    uint32_t *s_fm = reinterpret_cast<uint32_t *>(s_f);
+    uint32_t *s_fmi = reinterpret_cast<uint32_t *>(s_f + pad16(4 * (size_t)P));
+    uint16_t *s_fkey = reinterpret_cast<uint16_t *>(s_f + 2 * pad16(4 * (size_t)P));
+    if (BC.field) {  // uniform over the block
+      constexpr uint32_t nk = kGf4Keys * sizeof(uint16_t) / 16;
+      const u32x4 *ks = reinterpret_cast<const u32x4 *>(BC.T.fkey);
+      Staged<5, uint32_t> fm, fmi;
+      Staged<2, u32x4> fk;
+      fm.load(BC.T.fmat, P);
+      fmi.load(BC.T.finv, P);
+      fk.load(ks, nk);
+      ClsNarrowStage g;
+      g.run(BC.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
+      fm.store(s_fm, BC.T.fmat, P);
+      fmi.store(s_fmi, BC.T.finv, P);
+      fk.store(reinterpret_cast<u32x4 *>(s_fkey), ks, nk);
+    } else {
       ClsNarrowStage g;
       g.run(BC.T, s_inv, s_pool, s_tab0, s_rcq, s_tab, s_md);
     }
@@ -276,7 +319,19 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     const ClsCtx K{BC.T.retr, BC.cmargin};
     const ClsNarrow Lk{s_tab, s_tab0, s_rcq, s_md};
     uint32_t nfall = 0;
-    if (BC.bound && s_md) {  // uniform over the block
+    if (BC.field) {
+      const FieldView FV{s_fm, s_fmi, s_fkey};
+      for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x) {
+        ClsLink<D> S;
+        if (BC.bound && s_md) {
+          cls_prepare_field<D, true>(S, C, Lk, BC.G, FV, tk.mu, gmem(BC.sites)[tk.offset + t]);
+          acc += cls_hits<D, true>(S, C, Lk, K, T, tk.mu, nfall, nullptr);
+        } else {
+          cls_prepare_field<D, false>(S, C, Lk, BC.G, FV, tk.mu, gmem(BC.sites)[tk.offset + t]);
+          acc += cls_hits<D, false>(S, C, Lk, K, T, tk.mu, nfall, nullptr);
+        }
+      }
+    } else if (BC.bound && s_md) {  // uniform over the block
       for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)
         acc += update_link_cls<D, true>(C, Lk, K, BC.G, T, tk.mu, gmem(BC.sites)[tk.offset + t], nfall);
     } else {
@@ -584,26 +639,34 @@ size_t sweep_cls_lds_bytes(const Tables &T, int mode) {
 
 hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
   if (A.count == 0) return hipSuccess;
-  const size_t lds_small = sweep_cls_lds_bytes(A.T, kClsNarrow);
+  // small passes: the narrow small-pass kernel, with the GF(4) products
+  // where the chain runs the field form and its tables fit beside the draws
+  const bool fld = A.use_cls == kClsField && A.T.fmat;
+  const size_t lds_n = sweep_cls_lds_bytes(A.T, kClsNarrow), lds_f = sweep_cls_lds_bytes(A.T, kClsField);
+  const bool fit_f256 = fld && lds_f + cls_small_static_lds(256) <= A.lds_max;
+  const bool fit_f128 = fld && lds_f + cls_small_static_lds(128) <= A.lds_max;
+  const bool small_f = fit_f128 && A.count < kClsSmallPass ? true : fit_f256;  // field form fits this pass size
+  const size_t lds_small = small_f ? lds_f : lds_n;
   const bool sm = A.count < kClsSmallPass ||
                   lds_small + cls_small_static_lds(256) > A.lds_max;  // 128 links where 256 do not fit
   if ((A.use_cls == kClsNarrow || (A.use_cls == kClsField && !A.field_all)) && A.rng == kRngPhilox &&
-      A.count < kClsMidPass &&
-      lds_small + cls_small_static_lds(sm ? 128 : 256) <= A.lds_max) {
+      A.count < kClsMidPass && lds_small + cls_small_static_lds(sm ? 128 : 256) <= A.lds_max) {
     if (!A.fall || !A.T.cprod) return hipErrorInvalidValue;
     const uint32_t nl = sm ? 128 : 256, nb = (A.count + nl - 1) / nl;
     const size_t lds = lds_small;
     const bool bnd = A.bound && A.T.mdq;
+#define DSA_SMALL_F(d, FF)                                                                                     \
+    if (sm && bnd)                                                                                           \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 128, true, FF>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);    \
+    else if (sm)                                                                                             \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 128, false, FF>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);   \
+    else if (bnd)                                                                                            \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 256, true, FF>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);    \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_sweep_cls_small<d, 256, false, FF>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A);
 #define DSA_SMALL(d)                                                                                       \
   case d:                                                                                                \
-    if (sm && bnd)                                                                                       \
-      hipLaunchKernelGGL((k_sweep_cls_small<d, 128, true>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
-    else if (sm)                                                                                         \
-      hipLaunchKernelGGL((k_sweep_cls_small<d, 128, false>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
-    else if (bnd)                                                                                        \
-      hipLaunchKernelGGL((k_sweep_cls_small<d, 256, true>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
-    else                                                                                                 \
-      hipLaunchKernelGGL((k_sweep_cls_small<d, 256, false>), dim3(nb), dim3(kClsBlockNarrow), lds, s, A); \
+    if (small_f) { DSA_SMALL_F(d, true) } else { DSA_SMALL_F(d, false) }                                   \
     break;
     switch (A.G.D) {
       DSA_SMALL(2)
@@ -613,6 +676,7 @@ hipError_t launch_sweep(const SweepArgs &A, int max_blocks, hipStream_t s) {
       default: return hipErrorInvalidValue;
     }
 #undef DSA_SMALL
+#undef DSA_SMALL_F
     return hipGetLastError();
   }
   if (A.use_cls == kClsMatrix) {

@@ -106,6 +106,9 @@ struct BatchChain {
   double cmargin;
   unsigned long long *fall;
   int bound;  // as SweepArgs::bound
+  // with cls: the staple-times-link products from the group's GF(4) form
+  // (its tables staged after the class tables)
+  int field;
 };
 
 // The plaquette-first test is chosen for a sweep call when the chain's
@@ -161,6 +164,7 @@ struct LlrChain {
   // plaquette (1) and rectangle (2) parts; cmscale multiplies the bound
   int cls;
   double cmscale, ce1, ce2, cf1, cf2, cr1, cr2;
+  int field;  // with cls: the staple-times-link products from the GF(4) form
 };
 
 struct LlrState {
@@ -178,10 +182,13 @@ struct LlrState {
 };
 
 // LDS of the LLR pass kernel for one group: ReTr, inverses and pool, plus
-// (class path) the class of each element and the npool x P class table.
-__host__ __device__ inline size_t llr_lds_bytes(const Tables &T, bool cls) {
+// (class path) the class of each element and the npool x P class table,
+// plus (field) the GF(4) matrices, their inverses' and the key table.
+__host__ __device__ inline size_t llr_lds_bytes(const Tables &T, bool cls, bool field = false) {
   const size_t e = (size_t)T.P * sizeof(double) + (size_t)T.P * sizeof(uint16_t) + (size_t)T.npool * sizeof(uint16_t);
-  return cls ? pad16(e) + pad16(T.P) + pad16((size_t)T.npool * T.P) : e;
+  if (!cls) return e;
+  const size_t c = pad16(e) + pad16(T.P) + pad16((size_t)T.npool * T.P);
+  return field ? c + 2 * pad16(4 * (size_t)T.P) + 4096 * sizeof(uint16_t) : c;
 }
 
 // Static LDS of the LLR pass kernel (class values, the class path's uniforms).

@@ -157,9 +157,25 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
   uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_retr + P);
   uint16_t *s_pool = s_inv + P;
   // class path: class of each element and of pool[j] * t, after the element tables
-  const bool cls = LC.cls != 0;
+  const bool cls = LC.cls != 0, fld = cls && LC.field != 0;
   uint8_t *s_cls = smem + pad16(llr_lds_bytes(BC.T, false));
   uint8_t *s_cprod = s_cls + pad16(P);
+  // field: the GF(4) tables after the class tables
+  uint32_t *s_fm = reinterpret_cast<uint32_t *>(s_cprod + pad16((size_t)np * P));
+  uint32_t *s_fmi = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(s_fm) + pad16(4 * (size_t)P));
+  uint16_t *s_fkey = reinterpret_cast<uint16_t *>(reinterpret_cast<uint8_t *>(s_fmi) + pad16(4 * (size_t)P));
+  if (fld) {  // uniform over the block
+    constexpr uint32_t nk = kGf4Keys * sizeof(uint16_t) / 16;
+    const u32x4 *ks = reinterpret_cast<const u32x4 *>(BC.T.fkey);
+    Staged<5, uint32_t> fm, fmi;
+    Staged<2, u32x4> fk;
+    fm.load(BC.T.fmat, P);
+    fmi.load(BC.T.finv, P);
+    fk.load(ks, nk);
+    fm.store(s_fm, BC.T.fmat, P);
+    fmi.store(s_fmi, BC.T.finv, P);
+    fk.store(reinterpret_cast<u32x4 *>(s_fkey), ks, nk);
+  }
   {
     // every load of the block's staging issued before the first LDS store
     Staged<5, double> g_retr;
@@ -237,7 +253,10 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
       // t_k = s_k b, built directly: ReTr(b p s_k) = ReTr(t_k p), of the class
       // of pool[j] t_k; the element value ReTr(b s_k) is retr[b t_k b^-1]
       uint32_t tt[NL];
-      build_t<D>(BC.links, G, T, s, mu, b, tt);
+      if (fld)
+        build_t_field<D>(BC.links, G, FieldView{s_fm, s_fmi, s_fkey}, s, mu, b, tt);
+      else
+        build_t<D>(BC.links, G, T, s, mu, b, tt);
       auto sums = [&](ClsNarrow::Row row, int32_t &rc, int32_t &mc) {
         int32_t x = 0, y = 0;
 #pragma unroll

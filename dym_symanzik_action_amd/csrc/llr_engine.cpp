@@ -247,8 +247,14 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   // kernel's static arrays
   const bool cls = L->kernel != DSA_KERNEL_TABLE && T.cprod &&
                    dsa::llr_lds_bytes(T, true) + dsa::llr_pass_static_lds() <= L->lds_max;
-  if ((L->kernel == DSA_KERNEL_CLASS || L->kernel == DSA_KERNEL_CLASS_NARROW) && !cls)
+  if ((L->kernel == DSA_KERNEL_CLASS || L->kernel == DSA_KERNEL_CLASS_NARROW || L->kernel == DSA_KERNEL_FIELD) &&
+      !cls)
     return fail(DSA_E_STATE, "no class tables for this group on this device");
+  // with the GF(4) products where the group has that form and it fits too
+  const bool field = cls && T.fmat && (L->kernel == DSA_KERNEL_AUTO || L->kernel == DSA_KERNEL_FIELD) &&
+                     dsa::llr_lds_bytes(T, true, true) + dsa::llr_pass_static_lds() <= L->lds_max;
+  if (L->kernel == DSA_KERNEL_FIELD && !field)
+    return fail(DSA_E_STATE, "no GF(4) form of this group in this device's LDS");
   std::string err = dsa::build_colour_schedule(s.G, s.S);
   if (!err.empty()) return fail(DSA_E_STATE, err);
   const size_t nl = (size_t)s.G.V * s.G.D;
@@ -335,6 +341,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   d.cap = p->cap ? p->cap : 20000;
   d.npass = (uint32_t)s.S.passes.size();
   d.cls = cls ? 1 : 0;
+  d.field = field ? 1 : 0;
   d.cmscale = L->margin_scale;
   if (cls)  // byte tables: class values in units of 2^-24, half a unit of quantisation
     dsa::class_margin_terms(g->g, (int)p->D, 0x1p-25, d.ce1, d.ce2, d.cf1, d.cf2, d.cr1, d.cr2);
@@ -349,7 +356,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   s.used = true;
   L->slots[c] = s;
   L->used_bytes += bytes;
-  L->lds = std::max(L->lds, dsa::llr_lds_bytes(T, cls));
+  L->lds = std::max(L->lds, dsa::llr_lds_bytes(T, cls, field));
   L->dirty = true;
   if (slot_out) *slot_out = c;
   return 0;
@@ -358,7 +365,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
 int dsa_llr_set_kernel(dsa_llr *L, int32_t kernel, double margin_scale) {
   if (!L) return fail(DSA_E_ARG, "dsa_llr_set_kernel: NULL engine");
   if (kernel != DSA_KERNEL_AUTO && kernel != DSA_KERNEL_TABLE && kernel != DSA_KERNEL_CLASS &&
-      kernel != DSA_KERNEL_CLASS_NARROW)
+      kernel != DSA_KERNEL_CLASS_NARROW && kernel != DSA_KERNEL_FIELD)
     return fail(DSA_E_ARG, "dsa_llr_set_kernel: unknown kernel " + std::to_string(kernel));
   if (!(margin_scale >= 1.0) || !std::isfinite(margin_scale))
     return fail(DSA_E_ARG, "dsa_llr_set_kernel: margin_scale must be finite and >= 1");

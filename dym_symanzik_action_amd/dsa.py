@@ -408,8 +408,10 @@ class Batch:
         return idx.value
 
     def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
-        """Before finalize: "table", "class" / "class-narrow" or "auto" (the
-        class update where it fits); identical results."""
+        """Before finalize: "table", "class" / "class-narrow", "field" (the
+        class update with GF(4) products) or "auto" (the class update where it
+        fits, with GF(4) products where the group has the form); identical
+        results."""
         _check(lib().dsa_batch_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
 
     def exact_decisions(self) -> int:
@@ -560,9 +562,11 @@ class LlrEngine:
 
     def set_kernel(self, kernel: str = "auto", margin_scale: float = 1.0) -> None:
         """Pass kernel of chains added from now on: "table" (element values),
-        "class" / "class-narrow" (byte class tables in LDS, certified) or
-        "auto"; identical results.  A huge margin_scale decides every hit from
-        element values."""
+        "class" / "class-narrow" (byte class tables in LDS, certified),
+        "field" (the class path with the staple-times-link products from the
+        group's GF(4) form) or "auto" (the class path, with GF(4) products
+        where the group has the form); identical results.  A huge
+        margin_scale decides every hit from element values."""
         _check(lib().dsa_llr_set_kernel(self._h, DSA_KERNEL[kernel], margin_scale))
 
     def add(self, group: Group, params: LlrParams, links=None) -> int:

@@ -207,9 +207,11 @@ int dsa_batch_finalize(dsa_batch *b);
  * the reference; identical results either way).  TABLE: element values
  * (dym-mod-metro.cpp:164-192).  CLASS / CLASS_NARROW: the byte-index class
  * update of DSA_KERNEL_CLASS_NARROW for every chain (an error at finalize
- * for a group without class tables that fit).  AUTO (default): that update
- * for the chains where it fits and its decision bound is below 1e-3, the
- * table update for the others. */
+ * for a group without class tables that fit).  FIELD: that update with the
+ * GF(4) products of DSA_KERNEL_FIELD for every chain (an error at finalize
+ * for a group without that form).  AUTO (default): the class update for the
+ * chains where it fits and its decision bound is below 1e-3, with GF(4)
+ * products where the group has the form, the table update for the others. */
 int dsa_batch_set_kernel(dsa_batch *b, int32_t kernel, double margin_scale);
 /* Hits of the batch's class updates re-decided from element values. */
 int dsa_batch_exact_decisions(dsa_batch *b, uint64_t *n);
@@ -300,9 +302,12 @@ int dsa_llr_remove(dsa_llr *L, uint32_t slot);
  * gathers, as update_constrained sums them (llr.cpp:178-219).  CLASS or
  * CLASS_NARROW: the byte class tables in LDS with certified decisions, each
  * undecided hit decided again from element values; an error when the group
- * has no class tables that fit.  AUTO (default): the class path where it
- * fits.  margin_scale >= 1 widens the bound (a huge value decides every hit
- * from element values). */
+ * has no class tables that fit.  FIELD: the class path with the
+ * staple-times-link products from the group's GF(4) form (DSA_KERNEL_FIELD);
+ * an error when the group has no such form.  AUTO (default): the class path
+ * where it fits, with GF(4) products where the group has the form.
+ * margin_scale >= 1 widens the bound (a huge value decides every hit from
+ * element values). */
 int dsa_llr_set_kernel(dsa_llr *L, int32_t kernel, double margin_scale);
 /* nsweeps sweeps of every active chain (each in its own phase). */
 int dsa_llr_step(dsa_llr *L, uint32_t nsweeps);

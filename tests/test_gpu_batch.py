@@ -34,12 +34,15 @@ def build(dsa, dgroups, mix, kernel="auto", margin_scale=1.0):
     return b.finalize()
 
 
-@pytest.mark.parametrize("kernel", [("auto", 1.0), ("table", 1.0), ("auto", 1e300)],
-                         ids=["auto", "table", "class-all-exact"])
+@pytest.mark.parametrize("kernel", [("auto", 1.0), ("table", 1.0), ("auto", 1e300), ("class-narrow", 1.0),
+                                    ("field", 1.0)],
+                         ids=["auto", "table", "class-all-exact", "class-narrow", "field"])
 def test_batch_equals_lone_chains(dsa, dgroups, kernel):
     """Each chain equals the chain alone, with the class update where it fits
-    (auto), the table update everywhere, and the class update deciding every
-    hit from element values."""
+    (auto: with GF(4) products for S(1080) and S(216)), the table update
+    everywhere, the class update deciding every hit from element values, and
+    the class update with its products from table gathers or from the GF(4)
+    form for every chain."""
     say("case batch vs lone chains", kernel)
     nsw = 3
     with build(dsa, dgroups, MIX, *kernel) as b:

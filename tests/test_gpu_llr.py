@@ -45,14 +45,19 @@ def same(gpu, ora):
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[2]}x{c[3]}-axis{c[4]}-{c[10]}")
-@pytest.mark.parametrize("kernel", ["table", "class", "class-all-exact"])
+@pytest.mark.parametrize("kernel", ["table", "class", "class-all-exact", "field", "field-all-exact"])
 def test_llr_matches_oracle(dsa, dgroups, oracle, ogroups, case, kernel):
-    """Both pass kernels (element values; class values with certified
-    decisions, and with every hit sent to its exact path) against the oracle."""
+    """Every pass kernel (element values; class values with certified
+    decisions, with the staple-times-link products from table gathers or
+    from the GF(4) form, and with every hit sent to its exact path) against
+    the oracle."""
     say("case llr vs oracle", case, kernel)
     o = oracle_chain(oracle, ogroups, case)
     with dsa.dsa.LlrEngine(4, 4) as eng:
-        eng.set_kernel("class", 1e300) if kernel == "class-all-exact" else eng.set_kernel(kernel)
+        if kernel.endswith("-all-exact"):
+            eng.set_kernel(kernel[: -len("-all-exact")], 1e300)
+        else:
+            eng.set_kernel(kernel)
         s = eng.add(dgroups(case[0]), params(dsa, case))
         for sweep in range(30):
             eng.step(1)
@@ -147,12 +152,15 @@ def oracle2d(oracle, ogroups, c, links=None):
 
 
 @pytest.mark.parametrize("case", CASES2D, ids=lambda c: f"{c[0]}-{c[2]}x{c[3]}-{c[11]}")
-@pytest.mark.parametrize("kernel", ["table", "class", "class-all-exact"])
+@pytest.mark.parametrize("kernel", ["table", "class", "class-all-exact", "field", "field-all-exact"])
 def test_llr2d_matches_oracle(dsa, dgroups, oracle, ogroups, case, kernel):
     say("case llr2d vs oracle", case, kernel)
     o = oracle2d(oracle, ogroups, case)
     with dsa.dsa.LlrEngine(4, 2) as eng:
-        eng.set_kernel("class", 1e300) if kernel == "class-all-exact" else eng.set_kernel(kernel)
+        if kernel.endswith("-all-exact"):
+            eng.set_kernel(kernel[: -len("-all-exact")], 1e300)
+        else:
+            eng.set_kernel(kernel)
         s = eng.add(dgroups(case[0]), params2d(dsa, case))
         for sweep in range(30):
             eng.step(1)
