@@ -328,6 +328,15 @@ int cls_mode(const dsa_chain *c) {
   }
 }
 
+// The form the next sweep call runs: cls_mode, except that under AUTO a
+// frozen chain (acceptance below dsa::kBoundAcceptance over the previous
+// call, its links near one another) runs the class kernel with table
+// gathers, which then hit the caches, and its plaquette-first test.
+int run_mode(const dsa_chain *c) {
+  const int m = cls_mode(c);
+  return c->kernel == DSA_KERNEL_AUTO && c->bound_next && m == dsa::kClsField ? (int)dsa::kClsNarrow : m;
+}
+
 int ensure_lines(dsa_chain *c, uint32_t nt, uint32_t nx) {
   if (c->lines_valid && c->lines_t == nt && c->lines_x == nx) return 0;
   const size_t rows = (size_t)nt + (size_t)(c->G.D - 1) * nx;
@@ -541,7 +550,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.fall = c->d_fall;
   A.lds_max = c->lds_max;
   // the class form this chain runs (kClsNone: the table kernel)
-  const int mode = cls_mode(c);
+  const int mode = run_mode(c);
   A.cmargin = (mode == dsa::kClsMatrix ? c->mat_m0 : mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;
   A.cmargin_acc = c->mat_macc * c->margin_scale;
   A.bound = c->bound_next ? 1 : 0;
@@ -601,7 +610,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
 
 int dsa_chain_kernel_form(dsa_chain *c, int32_t *form) {
   if (!c || !form) return fail(DSA_E_ARG, "dsa_chain_kernel_form: NULL argument");
-  const int mode = cls_mode(c);
+  const int mode = run_mode(c);
   *form = mode == dsa::kClsMatrix  ? DSA_KERNEL_MATRIX
           : mode == dsa::kClsField ? DSA_KERNEL_FIELD
           : mode == dsa::kClsWide  ? DSA_KERNEL_CLASS

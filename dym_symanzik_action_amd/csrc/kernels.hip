@@ -295,7 +295,10 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     uint32_t *s_fm = reinterpret_cast<uint32_t *>(s_f);
     uint32_t *s_fmi = reinterpret_cast<uint32_t *>(s_f + pad16(4 * (size_t)P));
     uint16_t *s_fkey = reinterpret_cast<uint16_t *>(s_f + 2 * pad16(4 * (size_t)P));
-    if (BC.field) {  // uniform over the block
+    // a frozen chain (plaquette-first test on) keeps the table gathers, which
+    // hit the caches for links near one another
+    const bool fld = BC.field && !(BC.bound && s_md);
+    if (fld) {  // uniform over the block
       constexpr uint32_t nk = kGf4Keys * sizeof(uint16_t) / 16;
       const u32x4 *ks = reinterpret_cast<const u32x4 *>(BC.T.fkey);
       Staged<5, uint32_t> fm, fmi;
@@ -319,17 +322,12 @@ __global__ __launch_bounds__(kBatchBlockMax) void k_sweep_batch(const BatchChain
     const ClsCtx K{BC.T.retr, BC.cmargin};
     const ClsNarrow Lk{s_tab, s_tab0, s_rcq, s_md};
     uint32_t nfall = 0;
-    if (BC.field) {
+    if (fld) {
       const FieldView FV{s_fm, s_fmi, s_fkey};
       for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x) {
         ClsLink<D> S;
-        if (BC.bound && s_md) {
-          cls_prepare_field<D, true>(S, C, Lk, BC.G, FV, tk.mu, gmem(BC.sites)[tk.offset + t]);
-          acc += cls_hits<D, true>(S, C, Lk, K, T, tk.mu, nfall, nullptr);
-        } else {
-          cls_prepare_field<D, false>(S, C, Lk, BC.G, FV, tk.mu, gmem(BC.sites)[tk.offset + t]);
-          acc += cls_hits<D, false>(S, C, Lk, K, T, tk.mu, nfall, nullptr);
-        }
+        cls_prepare_field<D, false>(S, C, Lk, BC.G, FV, tk.mu, gmem(BC.sites)[tk.offset + t]);
+        acc += cls_hits<D, false>(S, C, Lk, K, T, tk.mu, nfall, nullptr);
       }
     } else if (BC.bound && s_md) {  // uniform over the block
       for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x)

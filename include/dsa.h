@@ -179,15 +179,18 @@ int dsa_chain_save_kentucky(dsa_chain *c, const char *prefix, int32_t num, char 
  * when the group is loaded, e.g. S(1080), S(216), S(108)): the byte-index
  * class update with its staple-times-link products formed as packed GF(4)
  * matrix products in registers instead of table gathers; AUTO takes it
- * wherever CLASS_NARROW would run and the form exists.
+ * wherever CLASS_NARROW would run and the form exists, except for a sweep
+ * call after one whose acceptance was below 1e-3 (a frozen chain: its links
+ * near one another make the table gathers cache hits, and the byte-index
+ * form then also tests each hit on its plaquette terms first).
  * margin_scale (default 1) multiplies the bound
  * that selects those re-decisions: a huge value makes every hit re-decided
  * (a test of that path). */
 enum { DSA_KERNEL_AUTO = 0, DSA_KERNEL_TABLE = 1, DSA_KERNEL_CLASS = 2, DSA_KERNEL_CLASS_NARROW = 3,
        DSA_KERNEL_MATRIX = 4, DSA_KERNEL_FIELD = 5 };
 int dsa_chain_set_kernel(dsa_chain *c, int32_t kernel, double margin_scale);
-/* The kernel the chain's passes run: TABLE, CLASS (the int16-value form),
- * CLASS_NARROW (the byte-index form), MATRIX or FIELD. */
+/* The kernel the chain's next sweep call runs: TABLE, CLASS (the int16-value
+ * form), CLASS_NARROW (the byte-index form), MATRIX or FIELD. */
 int dsa_chain_kernel_form(dsa_chain *c, int32_t *form);
 /* Hits the CLASS or MATRIX kernel re-decided from element values since creation. */
 int dsa_chain_exact_decisions(dsa_chain *c, uint64_t *n);
