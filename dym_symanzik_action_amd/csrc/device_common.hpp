@@ -270,57 +270,42 @@ struct FieldView {
 };
 
 // build_t with every product a packed GF(4) matrix product (no memory
-// access; the same 85 products).  All 62 links are loaded before the first
-// product, so a wavefront waits on global memory once per link-update.
-// Each link's matrix (or its inverse's) is then read from LDS and split
-// into the factor its one role needs (left or right); shared products are
-// split once.
+// access; the same 85 products).  Each link's matrix (or its inverse's) is
+// read from LDS and split into the factor its one role needs (left or
+// right); shared products are split once.  The loads stay where the
+// compiler places them (near their uses): issuing all 62 first needs more
+// than the 128 registers a lane has at four waves per SIMD, and the spills
+// cost more than the waits they save.
 template <int D>
 __device__ inline void build_t_field(const uint16_t *links, const Geom &G, const FieldView &F, const Site<D> &s,
                                      int mu, uint32_t b, uint32_t *t) {
   constexpr int NS = 2 * (D - 1);
-  // the links of each transverse direction nu, in the order used below
-  uint32_t e[D - 1][20];
-  const uint32_t e_mp = U<D>(links, G, s, mu, mu, 1, mu, 0);   // U_mu(x+mu)
-  const uint32_t e_mm = U<D>(links, G, s, mu, mu, -1, mu, 0);  // U_mu(x-mu)
+  const Gf4R Rb = F.R(b);
+  const Gf4L umu_p = F.L(U<D>(links, G, s, mu, mu, 1, mu, 0));                       // U_mu(x+mu)
+  const Gf4R C = gf4_right(gf4_mul(F.L(U<D>(links, G, s, mu, mu, -1, mu, 0)), Rb));   // U_mu(x-mu) b
 #pragma unroll
   for (int j = 0; j < D - 1; ++j) {
     const int nu = j + (j >= mu ? 1 : 0);
-    e[j][0] = U<D>(links, G, s, nu, mu, 1, nu, 0);
-    e[j][1] = U<D>(links, G, s, nu, mu, 0, nu, 0);
-    e[j][2] = U<D>(links, G, s, nu, mu, 1, nu, -1);
-    e[j][3] = U<D>(links, G, s, nu, mu, 0, nu, -1);
-    e[j][4] = U<D>(links, G, s, nu, mu, 1, nu, 1);
-    e[j][5] = U<D>(links, G, s, nu, mu, 0, nu, 1);
-    e[j][6] = U<D>(links, G, s, nu, mu, 2, nu, 0);
-    e[j][7] = U<D>(links, G, s, nu, mu, -1, nu, 0);
-    e[j][8] = U<D>(links, G, s, nu, mu, 1, nu, -2);
-    e[j][9] = U<D>(links, G, s, nu, mu, 0, nu, -2);
-    e[j][10] = U<D>(links, G, s, nu, mu, 2, nu, -1);
-    e[j][11] = U<D>(links, G, s, nu, mu, -1, nu, -1);
-    e[j][12] = U<D>(links, G, s, mu, mu, 0, nu, 1);
-    e[j][13] = U<D>(links, G, s, mu, mu, 0, nu, -1);
-    e[j][14] = U<D>(links, G, s, mu, mu, 0, nu, 2);
-    e[j][15] = U<D>(links, G, s, mu, mu, 1, nu, 1);
-    e[j][16] = U<D>(links, G, s, mu, mu, -1, nu, 1);
-    e[j][17] = U<D>(links, G, s, mu, mu, 0, nu, -2);
-    e[j][18] = U<D>(links, G, s, mu, mu, 1, nu, -1);
-    e[j][19] = U<D>(links, G, s, mu, mu, -1, nu, -1);
-  }
-  // keep every load above this point: the scheduler would otherwise sink
-  // each load to its first use and wait on memory once per batch
-  __builtin_amdgcn_sched_barrier(0);
-  const Gf4R Rb = F.R(b);
-  const Gf4L umu_p = F.L(e_mp);
-  const Gf4R C = gf4_right(gf4_mul(F.L(e_mm), Rb));  // U_mu(x-mu) b
-#pragma unroll
-  for (int j = 0; j < D - 1; ++j) {
-    const Gf4L n_10 = F.L(e[j][0]), n_00i = F.Li(e[j][1]), n_1mi = F.Li(e[j][2]), n_0m = F.L(e[j][3]);
-    const Gf4R n_11 = F.R(e[j][4]), n_01i = F.Ri(e[j][5]), n_20 = F.R(e[j][6]), n_m0i = F.Ri(e[j][7]);
-    const Gf4R n_1Mi = F.Ri(e[j][8]), n_0M = F.R(e[j][9]), n_2mi = F.Ri(e[j][10]), n_mm = F.R(e[j][11]);
-    const Gf4R m_01i = F.Ri(e[j][12]), m_0mi = F.Ri(e[j][13]);
-    const Gf4L m_02i = F.Li(e[j][14]), m_11i = F.Li(e[j][15]), m_m1i = F.Li(e[j][16]), m_0Mi = F.Li(e[j][17]);
-    const Gf4L m_1mi = F.Li(e[j][18]), m_mmi = F.Li(e[j][19]);
+    const Gf4L n_10 = F.L(U<D>(links, G, s, nu, mu, 1, nu, 0));
+    const Gf4L n_00i = F.Li(U<D>(links, G, s, nu, mu, 0, nu, 0));
+    const Gf4L n_1mi = F.Li(U<D>(links, G, s, nu, mu, 1, nu, -1));
+    const Gf4L n_0m = F.L(U<D>(links, G, s, nu, mu, 0, nu, -1));
+    const Gf4R n_11 = F.R(U<D>(links, G, s, nu, mu, 1, nu, 1));
+    const Gf4R n_01i = F.Ri(U<D>(links, G, s, nu, mu, 0, nu, 1));
+    const Gf4R n_20 = F.R(U<D>(links, G, s, nu, mu, 2, nu, 0));
+    const Gf4R n_m0i = F.Ri(U<D>(links, G, s, nu, mu, -1, nu, 0));
+    const Gf4R n_1Mi = F.Ri(U<D>(links, G, s, nu, mu, 1, nu, -2));
+    const Gf4R n_0M = F.R(U<D>(links, G, s, nu, mu, 0, nu, -2));
+    const Gf4R n_2mi = F.Ri(U<D>(links, G, s, nu, mu, 2, nu, -1));
+    const Gf4R n_mm = F.R(U<D>(links, G, s, nu, mu, -1, nu, -1));
+    const Gf4R m_01i = F.Ri(U<D>(links, G, s, mu, mu, 0, nu, 1));
+    const Gf4R m_0mi = F.Ri(U<D>(links, G, s, mu, mu, 0, nu, -1));
+    const Gf4L m_02i = F.Li(U<D>(links, G, s, mu, mu, 0, nu, 2));
+    const Gf4L m_11i = F.Li(U<D>(links, G, s, mu, mu, 1, nu, 1));
+    const Gf4L m_m1i = F.Li(U<D>(links, G, s, mu, mu, -1, nu, 1));
+    const Gf4L m_0Mi = F.Li(U<D>(links, G, s, mu, mu, 0, nu, -2));
+    const Gf4L m_1mi = F.Li(U<D>(links, G, s, mu, mu, 1, nu, -1));
+    const Gf4L m_mmi = F.Li(U<D>(links, G, s, mu, mu, -1, nu, -1));
     // level 1: products of two links (A, Bd with b), right or left forms
     // as their later products need them
     const Gf4R A = gf4_right(gf4_mul(n_00i, Rb));         // U_nu(x)^-1 b
