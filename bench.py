@@ -17,8 +17,13 @@ Prints one JSON line.  Besides the contract fields it carries
                 channel per clock) and a random-gather probe (table gathers
                 per link-update: 608 for the table kernel; 85 + 25 per
                 accepted hit + 98 per exact re-decision for the class
-                kernel; the probe's gathers each hit a random table row,
-                as the sweep's do);
+                kernel; 25 per accepted hit + 98 per re-decision for the
+                field kernel; the probe's gathers each hit a random table
+                row, as the sweep's do).  The field kernel builds its
+                staple products as GF(4) matrix products in registers, so
+                its limiter is the VALU issue rate instead: VALU wave-
+                instructions per launch from the committed counters over the
+                live launch time, against one per CU per shader clock;
   beta_scan     the 32^3 x 64 lattice at beta1 in {2.0, 2.5, 3.0, 3.5} x
                 beta2 in {0, -0.25}, cold and hot (seed 11): link-updates/s,
                 acceptance and exact re-decisions per hit, two sweeps each;
@@ -65,7 +70,10 @@ KERNEL_NAME = {"table": "k_sweep<4>", "class": "k_sweep_cls<4, true>", "class-na
                "matrix": "k_sweep_mat<4>", "field": "k_sweep_cls<4, false, false, true>"}
 HBM_PEAK_GBS = 8000.0
 L2_REQ_PEAK = 16 * 8 * 2.4e9   # L2 requests/s: 16 channels per XCD, 8 XCDs, 2.4 GHz
-PMC_FILE = {"table": "k_sweep_32x64_pmc.txt", "class-narrow": "k_sweep_cls_narrow_32x64_pmc.txt"}
+VALU_ISSUE_PER_CU = 1           # wave64 VALU instructions per CU per clock (4 SIMDs x 16 lanes)
+CUS = 256
+PMC_FILE = {"table": "k_sweep_32x64_pmc.txt", "class-narrow": "k_sweep_cls_narrow_32x64_pmc.txt",
+            "field": "k_sweep_cls_field_32x64_pmc.txt"}
 
 
 def log(*a):
@@ -347,6 +355,9 @@ def main():
     traffic = read_profile(form, "traffic_bytes_per_launch")
     l2_req = read_profile(form, "l2_read_requests_per_launch")
     l2_hit = read_profile(form, "l2_hit_rate")
+    valu = read_profile(form, "valu_insts_per_launch")
+    lds_active = read_profile(form, "lds_active_cycles_per_launch")
+    clock = read_profile(form, "shader_clock_ghz")
     if form == "table":
         gathers_per_unit = float(GATHERS_TABLE)
     elif form == "matrix":
@@ -364,6 +375,14 @@ def main():
         gathers_per_unit = GATHERS_CLASS + (GATHERS_CLASS_ACCEPT * (acc1 - acc0) +
                                             GATHERS_CLASS_EXACT * (ex1 - ex0)) / (args.steps * LINKS)
     gather_rate = gathers_per_unit * units / avg_launch_s
+    valu_issue = None
+    if valu and clock:
+        # VALU wave-instructions per second against one per CU per clock at
+        # the shader clock the counters saw; LDS busy share of the CU cycles
+        peak = CUS * VALU_ISSUE_PER_CU * clock * 1e9
+        valu_issue = {"achieved_per_s": valu / avg_launch_s, "peak_per_s": peak,
+                      "frac": valu / avg_launch_s / peak, "per_launch": valu,
+                      "lds_busy_frac": lds_active / (CUS * clock * 1e9 * avg_launch_s) if lds_active else None}
     gp = C.c_double()
     best = 0.0
     for blocks in (2048, 4096, 8192):
@@ -424,14 +443,18 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": KERNEL_NAME[form], "avg_launch_ms": avg_launch_s * 1e3, "units_per_launch": units,
                      "algorithmic_bytes_per_unit": BYTES_PER_LINK_UPDATE,
-                     # what actually bounds the sweep: its table gathers, served
-                     # by L2.  L1->L2 read requests (one 128-byte line each) per
-                     # launch from the committed rocprofv3 counters, over the
-                     # live launch time, against one request per L2 channel per
-                     # clock (16 channels x 8 XCDs x 2.4 GHz); and the gather
-                     # rate against the random-gather probe's
-                     "limiter": {"kind": "l2_requests",
-                                 "frac": l2_req / avg_launch_s / L2_REQ_PEAK if l2_req else None,
+                     # what actually bounds the sweep: for the gather forms its
+                     # table gathers, served by L2.  L1->L2 read requests (one
+                     # 128-byte line each) per launch from the committed
+                     # rocprofv3 counters, over the live launch time, against
+                     # one request per L2 channel per clock (16 channels x 8
+                     # XCDs x 2.4 GHz); and the gather rate against the
+                     # random-gather probe's.  For the field form the VALU
+                     # issue rate (the GF(4) products)
+                     "limiter": {"kind": "valu_issue" if form == "field" else "l2_requests",
+                                 "frac": ((valu_issue or {}).get("frac") if form == "field" else
+                                          l2_req / avg_launch_s / L2_REQ_PEAK if l2_req else None),
+                                 "valu_issue": valu_issue,
                                  "l2_requests": {"achieved_per_s": l2_req / avg_launch_s if l2_req else None,
                                                  "peak_per_s": L2_REQ_PEAK, "per_launch": l2_req,
                                                  "l2_hit_rate": l2_hit},
