@@ -492,13 +492,15 @@ __device__ inline uint32_t block_first(bool pred, uint32_t idx, uint32_t n, unsi
 // exclusive prefix sums of the changes, a run of acceptances from lane s, met
 // with energy E_s, has energy lo + o + q_k after lane k (o = E_s - lo - x_s),
 // so it ends at the first changed lane whose o + q_k leaves [0, hi - lo]: one
-// add and one compare per lane against uniform values.  A run of refusals
-// from a failing lane f leaves E unchanged and ends at the first changed lane
-// after f whose own change keeps E inside.  The walk follows the ruling
-// energies; the other energy and the hit count gain x_f - x_s per acceptance
-// run (read across lanes beside the walk's own reads); the lanes of the
-// acceptance runs are collected in a mask and the refused ones restore their
-// links.
+// add and one compare per lane against uniform values.  (Unchanged lanes
+// never end it: the energy after them is the energy before, inside.)  A run
+// of refusals from a failing lane f leaves E unchanged and ends at the first
+// changed lane after f whose own change keeps E inside.  The loop keeps only
+// what the next run needs (the ruling energies, the lanes still undecided)
+// and marks each run's first lane in T, so that a lane's outcome is the
+// parity of the marks at or below it: even for an acceptance run, odd for a
+// refusal run.  The other energy and the hits of the accepted proposals are
+// summed per lane over the chunks and across the wave once at the end.
 template <int AX>
 __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const long long *sP1, const long long *sP2,
                             const int *sPN, const uint8_t *sChg, const uint32_t *sPos, const uint16_t *sOld,
@@ -509,6 +511,11 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
   E1 = uniform64(E1);
   E2 = uniform64(E2);
   const unsigned long long Wa = (unsigned long long)(R.hi - R.lo), Wb = (unsigned long long)(R.hi2 - R.lo2);
+  const unsigned long long upto = ~0ull >> (63 - lane);  // lanes 0..lane
+  // the ruling energies minus lo, uniform over the wave
+  long long xa = (AX == 2 ? E2 : E1) - R.lo, xb = AX == 3 ? E2 - R.lo2 : 0;
+  long long so = 0;  // per lane: the other energy's gain by the accepted proposals
+  int sn = 0;        // per lane: hits of the accepted proposals
   for (uint32_t base = f0 & ~63u; base < n; base += 64) {  // n <= kDecideSeg: base + 63 < kDecideSeg
     const uint32_t cn = n - base < 64 ? n - base : 64;
     const uint32_t kk = base + (uint32_t)lane, kp = kk ? kk - 1 : 0;
@@ -520,59 +527,61 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
     const long long x1 = kk ? sP1[kp] : 0, x2 = kk ? sP2[kp] : 0;
     const int qn = sPN[kk], xn = kk ? sPN[kp] : 0;
     const uint8_t ch = sChg[kk];
-    const unsigned long long cm = __ballot(valid & (ch != 0));  // changed lanes
-    const uint32_t s_init = base < f0 ? f0 - base : 0;          // lanes before it are settled
+    const uint32_t s_init = base < f0 ? f0 - base : 0;  // lanes before it are settled
     // the ruling energy: S1 on axes 1 and 3, S2 on axis 2; the box adds S2;
-    // the other one (axes 1, 2) is summed by runs
+    // the other one (axes 1, 2) is summed per lane
     const long long qa = AX == 2 ? q2 : q1, xa_ = AX == 2 ? x2 : x1;
-    const long long qo = AX == 2 ? q1 : q2, xo = AX == 2 ? x1 : x2;
     const long long ea = qa - xa_, eb = q2 - x2;
-    long long so = 0;  // gain of the other energy over the chunk
-    int sn = 0;        // hits of the accepted changed proposals
-    uint32_t s = s_init;  // uniform over the wave, < 64
-    long long o = (AX == 2 ? E2 : E1) - R.lo - readlane64(xa_, s);
-    long long ob = AX == 3 ? E2 - R.lo2 - readlane64(x2, s) : 0;
-    long long xa, xb = 0;  // the energies minus lo at the chunk's end
-    unsigned long long amask = 0;
+    // changed lanes not yet decided, uniform over the wave
+    unsigned long long rem = __ballot(valid & (ch != 0)) & (~0ull << s_init);
+    unsigned long long T = 0;
+    long long o = xa - readlane64(xa_, s_init);
+    long long ob = AX == 3 ? xb - readlane64(x2, s_init) : 0;
+    bool tail_refused = false;
     for (;;) {
       bool out = (unsigned long long)(qa + o) > Wa;
       if (AX == 3) out = out | ((unsigned long long)(q2 + ob) > Wb);
-      const unsigned long long from_s = ~0ull << s;
-      const unsigned long long fm = __ballot(out) & cm & from_s;
-      const long long os = readlane64(xo, s);  // before the run
-      const int ns = __builtin_amdgcn_readlane(xn, (int)s);
-      if (!fm) {  // the rest of the chunk is accepted
-        amask |= from_s;
-        xa = o + readlane64(qa, cn - 1);
-        if (AX == 3) xb = ob + readlane64(q2, cn - 1);
-        if (AX != 3) so += readlane64(qo, cn - 1) - os;
-        sn += __builtin_amdgcn_readlane(qn, (int)(cn - 1)) - ns;
-        break;
-      }
+      const unsigned long long fm = __ballot(out) & rem;
+      if (!fm) break;  // the rest of the chunk is accepted
       const uint32_t f = (uint32_t)__builtin_ctzll(fm);
-      amask |= from_s & ((1ull << f) - 1);
-      xa = o + readlane64(xa_, f);  // the energy met by lane f
-      if (AX == 3) xb = ob + readlane64(x2, f);
-      if (AX != 3) so += readlane64(xo, f) - os;
-      sn += __builtin_amdgcn_readlane(xn, (int)f) - ns;
+      T |= 1ull << f;
+      xa = readlane64(o + xa_, f);  // the energy met by lane f
+      if (AX == 3) xb = readlane64(ob + x2, f);
       bool in = (unsigned long long)(ea + xa) <= Wa;
       if (AX == 3) in = in & ((unsigned long long)(eb + xb) <= Wb);
-      const unsigned long long gm = __ballot(in) & cm & (f < 63 ? ~0ull << (f + 1) : 0ull);
-      if (!gm) break;  // the rest of the chunk is refused: the energy stays
+      rem &= ~1ull << f;  // the lanes after f (none for f = 63)
+      const unsigned long long gm = __ballot(in) & rem;
+      if (!gm) {  // the rest of the chunk is refused: the energy stays
+        tail_refused = true;
+        break;
+      }
       const uint32_t g = (uint32_t)__builtin_ctzll(gm);
+      T |= 1ull << g;
+      rem &= ~0ull << g;
       o = xa - readlane64(xa_, g);
       if (AX == 3) ob = xb - readlane64(x2, g);
-      s = g;
     }
-    const bool took = ((amask >> lane) & 1ull) != 0;
-    if ((uint32_t)lane >= s_init && ((cm >> lane) & 1ull) && !took) gmem(links)[sPos[kk]] = sOld[kk];
-    if (AX == 1) E2 += so;
-    if (AX == 2) E1 += so;
-    if (AX != 2) E1 = xa + R.lo;
-    if (AX == 2) E2 = xa + R.lo;
-    if (AX == 3) E2 = xb + R.lo2;
-    acc += (unsigned long long)(unsigned)sn;
+    if (!tail_refused) {
+      xa = o + readlane64(qa, cn - 1);
+      if (AX == 3) xb = ob + readlane64(q2, cn - 1);
+    }
+    const bool refused = (__popcll(T & upto) & 1) != 0;
+    const bool mine = valid && (uint32_t)lane >= s_init;
+    if (mine && ch && refused) gmem(links)[sPos[kk]] = sOld[kk];
+    if (mine && !refused) {  // unchanged lanes add nothing
+      if (AX == 1) so += q2 - x2;
+      if (AX == 2) so += q1 - x1;
+      sn += qn - xn;
+    }
   }
+  if (AX != 3) so = uniform64(wave_sum_i64(so));
+  const long long hits = uniform64(wave_sum_i64(sn));
+  if (AX == 1) E2 += so;
+  if (AX == 2) E1 += so;
+  if (AX != 2) E1 = xa + R.lo;
+  if (AX == 2) E2 = xa + R.lo;
+  if (AX == 3) E2 = xb + R.lo2;
+  acc += (unsigned long long)hits;
 }
 
 __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
