@@ -200,7 +200,9 @@ hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const Ba
 // Per chain (one wave each): accept the pass's link proposals in site order
 // while the running energy obeys the window (or, in drive-in, approaches it),
 // restoring the links whose proposal is refused.
-hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, uint32_t n, uint32_t pass, hipStream_t s);
+// one block per chain idx[0 .. n) (chains 0 .. n when idx is null)
+hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n, uint32_t pass,
+                             hipStream_t s);
 hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n, hipStream_t s);
 hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, uint32_t n, const ObsTask *tasks,
                              uint32_t ntasks, hipStream_t s);

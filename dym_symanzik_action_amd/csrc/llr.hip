@@ -516,17 +516,45 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
   long long xa = (AX == 2 ? E2 : E1) - R.lo, xb = AX == 3 ? E2 - R.lo2 : 0;
   long long so = 0;  // per lane: the other energy's gain by the accepted proposals
   int sn = 0;        // per lane: hits of the accepted proposals
+  // a chunk's values, loaded while the chunk before it is walked: inclusive
+  // (q) and exclusive (x) prefix sums (lane 0's exclusive ones being the
+  // sums before the chunk), the changed flag and the link to restore.  Every
+  // load is in bounds and unconditional, so that they wait together; lanes
+  // past the segment read values that are never used.
+  struct Chunk {
+    long long q1, q2, x1, x2;
+    int qn, xn;
+    uint32_t ch, pos, old;
+  };
+  auto load_chunk = [&](uint32_t b) {
+    Chunk c;
+    const uint32_t k = b + (uint32_t)lane < (uint32_t)kDecideSeg ? b + (uint32_t)lane : (uint32_t)kDecideSeg - 1;
+    const uint32_t kp = k ? k - 1 : 0;
+    c.q1 = sP1[k];
+    c.q2 = sP2[k];
+    c.x1 = sP1[kp];
+    c.x2 = sP2[kp];
+    c.qn = sPN[k];
+    c.xn = sPN[kp];
+    c.ch = sChg[k];
+    c.pos = sPos[k];
+    c.old = sOld[k];
+    if (!k) {
+      c.x1 = 0;
+      c.x2 = 0;
+      c.xn = 0;
+    }
+    return c;
+  };
+  Chunk nxt = load_chunk(f0 & ~63u);
   for (uint32_t base = f0 & ~63u; base < n; base += 64) {  // n <= kDecideSeg: base + 63 < kDecideSeg
     const uint32_t cn = n - base < 64 ? n - base : 64;
-    const uint32_t kk = base + (uint32_t)lane, kp = kk ? kk - 1 : 0;
     const bool valid = (uint32_t)lane < cn;
-    // every load in bounds and unconditional, so that they wait together:
-    // inclusive (q) and exclusive (x) prefix sums, lane 0's exclusive ones
-    // being the sums before the chunk
-    const long long q1 = sP1[kk], q2 = sP2[kk];
-    const long long x1 = kk ? sP1[kp] : 0, x2 = kk ? sP2[kp] : 0;
-    const int qn = sPN[kk], xn = kk ? sPN[kp] : 0;
-    const uint8_t ch = sChg[kk];
+    const Chunk cur = nxt;
+    nxt = load_chunk(base + 64);
+    const long long q1 = cur.q1, q2 = cur.q2, x1 = cur.x1, x2 = cur.x2;
+    const int qn = cur.qn, xn = cur.xn;
+    const uint32_t ch = cur.ch;
     const uint32_t s_init = base < f0 ? f0 - base : 0;  // lanes before it are settled
     // the ruling energy: S1 on axes 1 and 3, S2 on axis 2; the box adds S2;
     // the other one (axes 1, 2) is summed per lane
@@ -567,7 +595,7 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
     }
     const bool refused = (__popcll(T & upto) & 1) != 0;
     const bool mine = valid && (uint32_t)lane >= s_init;
-    if (mine && ch && refused) gmem(links)[sPos[kk]] = sOld[kk];
+    if (mine && ch && refused) gmem(links)[cur.pos] = (uint16_t)cur.old;
     if (mine && !refused) {  // unchanged lanes add nothing
       if (AX == 1) so += q2 - x2;
       if (AX == 2) so += q1 - x1;
@@ -584,7 +612,8 @@ __device__ void walk_window(const LlrRule &R, uint32_t n, uint32_t f0, const lon
   acc += (unsigned long long)hits;
 }
 
-__global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, LlrState *st, uint32_t pass) {
+__global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, LlrState *st, const uint32_t *idx,
+                                                        uint32_t pass) {
   __shared__ long long sP1[kDecideSeg], sP2[kDecideSeg];
   __shared__ int sPN[kDecideSeg];
   __shared__ uint32_t sPos[kDecideSeg];
@@ -596,7 +625,7 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
   __shared__ long long sE[2];
   __shared__ unsigned long long sAcc;
   static_assert(kDecideWaves <= 16, "the waves' totals fit one row of 16 lanes");
-  const uint32_t c = blockIdx.x;
+  const uint32_t c = idx ? gmem(idx)[blockIdx.x] : blockIdx.x;
   const uint32_t tid = threadIdx.x;
   const int lane = (int)(tid & 63u), wv = (int)(tid >> 6);
   const LlrChain &LC = chains[c];
@@ -949,9 +978,10 @@ hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const Ba
   return hipGetLastError();
 }
 
-hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, uint32_t n, uint32_t pass, hipStream_t s) {
+hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n, uint32_t pass,
+                             hipStream_t s) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_llr_decide, dim3(n), dim3(kDecide), 0, s, chains, st, pass);
+  hipLaunchKernelGGL(k_llr_decide, dim3(n), dim3(kDecide), 0, s, chains, st, idx, pass);
   return hipGetLastError();
 }
 

@@ -1,14 +1,23 @@
 // llr_engine.cpp -- device-resident LLR chains (include/dsa.h, dsa_llr_*):
-// admission into slots, one launch per pass over every active chain, the
-// per-chain drive-in / Robbins-Monro state machine on the device, and a
+// admission into slots, the active chains split into two groups of about
+// equal links, each group's passes launched on a stream of its own (one
+// proposal launch and one decision launch per pass over the group's chains),
+// the per-chain drive-in / Robbins-Monro state machine on the device, and a
 // continuous-batching serving loop for streams of window requests.
+//
+// The decision launch walks each chain's proposals in order with one
+// wavefront per chain, which leaves most of the device idle; with two
+// groups the other group's proposal launch runs beside it.  Chains are
+// independent, so the grouping does not change any chain's results.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <chrono>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -25,24 +34,40 @@ constexpr uint32_t kTaskLinks = 256;
 constexpr uint32_t kTaskLinksSmall = 64;  // for passes below kSmallPass links in all
 constexpr uint64_t kSmallPass = 32768;
 constexpr uint32_t kObsSites = 1024;
+constexpr int kGroups = 2;
+
+// What every chain of one lattice shape shares: the colour schedule (a
+// 16^4 one takes tens of milliseconds to build and verify on the host), the
+// position of each scheduled link in the link array, and their device
+// copies, which the kernels only read.  Built at the first admission of the
+// shape and kept until the engine is destroyed.
+struct Layout {
+  std::array<unsigned, 3> shape{};  // D, Nt, Nx
+  dsa::Schedule S;
+  uint32_t *sites = nullptr, *pos = nullptr;
+  dsa::PassRange *passes = nullptr;
+  ~Layout() {
+    for (void *p : {(void *)sites, (void *)pos, (void *)passes})
+      if (p) (void)hipFree(p);
+  }
+};
 
 struct Slot {
   bool used = false;
   dsa_group *grp = nullptr;
   dsa_llr_params p{};
   dsa::Geom G{};
-  dsa::Schedule S;
+  std::shared_ptr<const Layout> lay;
   dsa::DeviceTables dt;
   uint16_t *links = nullptr, *old = nullptr;
-  uint32_t *sites = nullptr, *pos = nullptr;
   long long *dE1 = nullptr, *dE2 = nullptr;
   uint8_t *nacc = nullptr;
-  dsa::PassRange *passes = nullptr;
   uint64_t bytes = 0;
 };
 
 // links, previous values, site lists, link positions, two energy changes and
-// a hit count per link
+// a hit count per link (the site lists and positions are shared by the chains
+// of one shape, so this overstates what a chain adds once its shape is in)
 uint64_t footprint(const dsa_llr_params &p) {
   uint64_t V = p.Nt;
   for (uint32_t d = 1; d < p.D; ++d) V *= p.Nx;
@@ -59,19 +84,27 @@ struct dsa_llr {
   uint32_t cap = 0;
   uint64_t budget = 0, used_bytes = 0;
   std::vector<Slot> slots;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;         // set-up, sweep ends, copies
+  hipStream_t gstream[kGroups] = {};    // each group's passes
+  hipEvent_t fork = nullptr, stagger = nullptr, join[kGroups] = {};
   dsa::LlrChain *d_chains = nullptr;
   dsa::LlrState *d_state = nullptr;
   dsa::BatchTask *d_tasks = nullptr;
   dsa::ObsTask *d_obs = nullptr;
   size_t tasks_cap = 0, obs_cap = 0;
-  std::vector<uint32_t> pass_off, pass_n;
+  // group g's tasks of pass q: [pass_off[g][q], + pass_n[g][q]) of d_tasks;
+  // its chains: d_idx[idx_off[g] .. + idx_n[g])
+  std::vector<uint32_t> pass_off[kGroups], pass_n[kGroups];
+  uint32_t *d_idx = nullptr;
+  size_t idx_cap = 0;
+  uint32_t idx_off[kGroups] = {}, idx_n[kGroups] = {};
   uint32_t nobs = 0;
   size_t lds = 0, lds_max = 0;
   int kernel = DSA_KERNEL_AUTO;  // for chains added from now on
   double margin_scale = 1.0;
   bool dirty = true;
   std::vector<dsa::LlrState> host_state;
+  std::vector<std::shared_ptr<const Layout>> layouts;
 };
 
 namespace {
@@ -79,27 +112,47 @@ namespace {
 int rebuild_tasks(dsa_llr *L) {
   std::vector<dsa::BatchTask> tasks;
   std::vector<dsa::ObsTask> obs;
-  size_t maxpass = 0;
-  for (const Slot &s : L->slots)
-    if (s.used) maxpass = std::max(maxpass, s.S.passes.size());
-  L->pass_off.clear();
-  L->pass_n.clear();
-  for (size_t q = 0; q < maxpass; ++q) {
-    L->pass_off.push_back((uint32_t)tasks.size());
-    // a pass that would not fill the device is cut into blocks of fewer
-    // links, so that more lanes share each link's random draws
-    uint64_t total = 0;
-    for (const Slot &s : L->slots)
-      if (s.used && q < s.S.passes.size()) total += s.S.passes[q].count;
-    const uint32_t per = total < kSmallPass ? kTaskLinksSmall : kTaskLinks;
-    for (uint32_t c = 0; c < L->slots.size(); ++c) {
-      const Slot &s = L->slots[c];
-      if (!s.used || q >= s.S.passes.size()) continue;
-      const dsa::Pass &ps = s.S.passes[q];
-      for (uint32_t o = 0; o < ps.count; o += per)
-        tasks.push_back(dsa::BatchTask{c, ps.offset + o, std::min(per, ps.count - o), ps.mu});
+  // the groups: chains by links, largest first, each to the group with
+  // fewer links so far
+  std::vector<uint32_t> act;
+  for (uint32_t c = 0; c < L->slots.size(); ++c)
+    if (L->slots[c].used) act.push_back(c);
+  auto links_of = [&](uint32_t c) { return (uint64_t)L->slots[c].G.V * L->slots[c].G.D; };
+  std::stable_sort(act.begin(), act.end(), [&](uint32_t x, uint32_t y) { return links_of(x) > links_of(y); });
+  std::vector<uint32_t> mem[kGroups];
+  uint64_t load[kGroups] = {};
+  for (uint32_t c : act) {
+    const int g = load[1] < load[0] ? 1 : 0;
+    mem[g].push_back(c);
+    load[g] += links_of(c);
+  }
+  std::vector<uint32_t> idx;
+  for (int g = 0; g < kGroups; ++g) {
+    std::sort(mem[g].begin(), mem[g].end());
+    L->idx_off[g] = (uint32_t)idx.size();
+    L->idx_n[g] = (uint32_t)mem[g].size();
+    idx.insert(idx.end(), mem[g].begin(), mem[g].end());
+    size_t maxpass = 0;
+    for (uint32_t c : mem[g]) maxpass = std::max(maxpass, L->slots[c].lay->S.passes.size());
+    L->pass_off[g].clear();
+    L->pass_n[g].clear();
+    for (size_t q = 0; q < maxpass; ++q) {
+      L->pass_off[g].push_back((uint32_t)tasks.size());
+      // a pass that would not fill the device is cut into blocks of fewer
+      // links, so that more lanes share each link's random draws
+      uint64_t total = 0;
+      for (uint32_t c : mem[g])
+        if (q < L->slots[c].lay->S.passes.size()) total += L->slots[c].lay->S.passes[q].count;
+      const uint32_t per = total < kSmallPass ? kTaskLinksSmall : kTaskLinks;
+      for (uint32_t c : mem[g]) {
+        const Slot &s = L->slots[c];
+        if (q >= s.lay->S.passes.size()) continue;
+        const dsa::Pass &ps = s.lay->S.passes[q];
+        for (uint32_t o = 0; o < ps.count; o += per)
+          tasks.push_back(dsa::BatchTask{c, ps.offset + o, std::min(per, ps.count - o), ps.mu});
+      }
+      L->pass_n[g].push_back((uint32_t)tasks.size() - L->pass_off[g].back());
     }
-    L->pass_n.push_back((uint32_t)tasks.size() - L->pass_off.back());
   }
   for (uint32_t c = 0; c < L->slots.size(); ++c) {
     const Slot &s = L->slots[c];
@@ -113,6 +166,12 @@ int rebuild_tasks(dsa_llr *L) {
     L->tasks_cap = tasks.size() * 2;
     HIP_TRY(hipMalloc(&L->d_tasks, L->tasks_cap * sizeof(dsa::BatchTask)));
   }
+  if (idx.size() > L->idx_cap) {
+    if (L->d_idx) HIP_TRY(hipFree(L->d_idx));
+    L->d_idx = nullptr;
+    L->idx_cap = std::max<size_t>(L->slots.size(), 1);
+    HIP_TRY(hipMalloc(&L->d_idx, L->idx_cap * sizeof(uint32_t)));
+  }
   if (obs.size() > L->obs_cap) {
     if (L->d_obs) HIP_TRY(hipFree(L->d_obs));
     L->d_obs = nullptr;
@@ -122,6 +181,8 @@ int rebuild_tasks(dsa_llr *L) {
   if (!tasks.empty())
     HIP_TRY(hipMemcpyAsync(L->d_tasks, tasks.data(), tasks.size() * sizeof(dsa::BatchTask), hipMemcpyHostToDevice,
                            L->stream));
+  if (!idx.empty())
+    HIP_TRY(hipMemcpyAsync(L->d_idx, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice, L->stream));
   if (!obs.empty())
     HIP_TRY(hipMemcpyAsync(L->d_obs, obs.data(), obs.size() * sizeof(dsa::ObsTask), hipMemcpyHostToDevice,
                            L->stream));
@@ -132,8 +193,7 @@ int rebuild_tasks(dsa_llr *L) {
 }
 
 void release(Slot &s) {
-  for (void *p : {(void *)s.links, (void *)s.old, (void *)s.sites, (void *)s.pos, (void *)s.dE1, (void *)s.dE2,
-                  (void *)s.nacc, (void *)s.passes})
+  for (void *p : {(void *)s.links, (void *)s.old, (void *)s.dE1, (void *)s.dE2, (void *)s.nacc})
     if (p) (void)hipFree(p);
   s = Slot();
 }
@@ -144,10 +204,30 @@ int step(dsa_llr *L, uint32_t n) {
   const uint32_t nc = (uint32_t)L->slots.size();
   HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nc, L->d_obs, L->nobs, L->stream));
   for (uint32_t it = 0; it < n; ++it) {
-    for (size_t q = 0; q < L->pass_off.size(); ++q) {
-      const dsa::BatchTask *tk = L->d_tasks + L->pass_off[q];
-      HIP_TRY(dsa::launch_llr_pass(L->D, L->d_chains, L->d_state, tk, L->pass_n[q], L->lds, L->stream));
-      HIP_TRY(dsa::launch_llr_decide(L->d_chains, L->d_state, nc, (uint32_t)q, L->stream));
+    // each group's 32 (or so) passes on its own stream, after the previous
+    // sweep's end; the second group starts once the first group's first
+    // proposal launch is done, so that from then on each group's decision
+    // launch tends to run beside the other's proposal launch; the sweep ends
+    // once both groups are through
+    HIP_TRY(hipEventRecord(L->fork, L->stream));
+    HIP_TRY(hipStreamWaitEvent(L->gstream[0], L->fork, 0));
+    const bool stagger = !L->pass_off[0].empty() && !L->pass_off[1].empty();
+    if (!stagger) HIP_TRY(hipStreamWaitEvent(L->gstream[1], L->fork, 0));
+    size_t np = 0;
+    for (int g = 0; g < kGroups; ++g) np = std::max(np, L->pass_off[g].size());
+    for (size_t q = 0; q < np; ++q)
+      for (int g = 0; g < kGroups; ++g) {
+        if (q >= L->pass_off[g].size()) continue;
+        const dsa::BatchTask *tk = L->d_tasks + L->pass_off[g][q];
+        if (g == 1 && q == 0 && stagger) HIP_TRY(hipStreamWaitEvent(L->gstream[1], L->stagger, 0));
+        HIP_TRY(dsa::launch_llr_pass(L->D, L->d_chains, L->d_state, tk, L->pass_n[g][q], L->lds, L->gstream[g]));
+        if (g == 0 && q == 0 && stagger) HIP_TRY(hipEventRecord(L->stagger, L->gstream[0]));
+        HIP_TRY(dsa::launch_llr_decide(L->d_chains, L->d_state, L->d_idx + L->idx_off[g], L->idx_n[g], (uint32_t)q,
+                                       L->gstream[g]));
+      }
+    for (int g = 0; g < kGroups; ++g) {
+      HIP_TRY(hipEventRecord(L->join[g], L->gstream[g]));
+      HIP_TRY(hipStreamWaitEvent(L->stream, L->join[g], 0));
     }
     HIP_TRY(dsa::launch_llr_sweep_end(L->d_chains, L->d_state, nc, L->stream));
     HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nc, L->d_obs, L->nobs, L->stream));
@@ -192,8 +272,13 @@ int dsa_llr_create(uint32_t D, uint32_t max_chains, uint64_t hbm_budget, dsa_llr
   if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, L->device) != hipSuccess)
     lds_max = 0;
   L->lds_max = (size_t)lds_max;
-  if (hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&L->d_chains, max_chains * sizeof(dsa::LlrChain)) != hipSuccess ||
+  bool streams_ok = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&L->stagger, hipEventDisableTiming) == hipSuccess;
+  for (int g = 0; g < kGroups && streams_ok; ++g)
+    streams_ok = hipStreamCreateWithFlags(&L->gstream[g], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&L->join[g], hipEventDisableTiming) == hipSuccess;
+  if (!streams_ok || hipMalloc(&L->d_chains, max_chains * sizeof(dsa::LlrChain)) != hipSuccess ||
       hipMalloc(&L->d_state, max_chains * sizeof(dsa::LlrState)) != hipSuccess)
     return bail(fail(DSA_E_DEVICE, "allocation failed"));
   std::vector<dsa::LlrState> idle(max_chains);
@@ -212,9 +297,17 @@ void dsa_llr_destroy(dsa_llr *L) {
   if (!L) return;
   (void)hipSetDevice(L->device);
   if (L->stream) (void)hipStreamSynchronize(L->stream);
+  for (int g = 0; g < kGroups; ++g)
+    if (L->gstream[g]) (void)hipStreamSynchronize(L->gstream[g]);
   for (Slot &s : L->slots) release(s);
-  for (void *p : {(void *)L->d_chains, (void *)L->d_state, (void *)L->d_tasks, (void *)L->d_obs})
+  for (void *p : {(void *)L->d_chains, (void *)L->d_state, (void *)L->d_tasks, (void *)L->d_obs, (void *)L->d_idx})
     if (p) (void)hipFree(p);
+  for (int g = 0; g < kGroups; ++g) {
+    if (L->gstream[g]) (void)hipStreamDestroy(L->gstream[g]);
+    if (L->join[g]) (void)hipEventDestroy(L->join[g]);
+  }
+  if (L->fork) (void)hipEventDestroy(L->fork);
+  if (L->stagger) (void)hipEventDestroy(L->stagger);
   if (L->stream) (void)hipStreamDestroy(L->stream);
   delete L;
 }
@@ -255,41 +348,52 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
                      dsa::llr_lds_bytes(T, true, true) + dsa::llr_pass_static_lds() <= L->lds_max;
   if (L->kernel == DSA_KERNEL_FIELD && !field)
     return fail(DSA_E_STATE, "no GF(4) form of this group in this device's LDS");
-  std::string err = dsa::build_colour_schedule(s.G, s.S);
-  if (!err.empty()) return fail(DSA_E_STATE, err);
+  const std::array<unsigned, 3> shape{p->D, p->Nt, p->Nx};
+  for (const auto &l : L->layouts)
+    if (l->shape == shape) s.lay = l;
+  if (!s.lay) {
+    auto lay = std::make_shared<Layout>();
+    lay->shape = shape;
+    std::string err = dsa::build_colour_schedule(s.G, lay->S);
+    if (!err.empty()) return fail(DSA_E_STATE, err);
+    const dsa::Schedule &S = lay->S;
+    const size_t ns = S.sites.size();
+    std::vector<dsa::PassRange> pr;
+    for (const dsa::Pass &ps : S.passes) pr.push_back(dsa::PassRange{ps.offset, ps.count, ps.mu, 0});
+    // index of each scheduled link in the per-direction, per-parity link array
+    std::vector<uint32_t> lpos(ns);
+    for (const dsa::Pass &ps : S.passes)
+      for (uint32_t q = ps.offset; q < ps.offset + ps.count; ++q) {
+        const uint32_t site = S.sites[q];
+        uint32_t par = 0, r = site;
+        for (int d = 0; d < s.G.D; ++d) {
+          par += r % s.G.ext[d];
+          r /= s.G.ext[d];
+        }
+        const uint32_t at = s.G.eo ? ((par & 1u) ? s.G.halfV : 0u) + (site >> 1) : site;
+        lpos[q] = (uint32_t)ps.mu * s.G.V + at;
+      }
+    if (hipMalloc(&lay->sites, ns * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&lay->pos, ns * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&lay->passes, pr.size() * sizeof(dsa::PassRange)) != hipSuccess)
+      return fail(DSA_E_DEVICE, "layout allocation failed");
+    if (hipMemcpy(lay->sites, S.sites.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(lay->pos, lpos.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(lay->passes, pr.data(), pr.size() * sizeof(dsa::PassRange), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(DSA_E_DEVICE, "layout upload failed");
+    L->layouts.push_back(lay);
+    s.lay = lay;
+  }
   const size_t nl = (size_t)s.G.V * s.G.D;
   auto cleanup = [&](int rc) {
     release(s);
     return rc;
   };
-  const size_t ns = s.S.sites.size();
-  std::vector<dsa::PassRange> pr;
-  for (const dsa::Pass &ps : s.S.passes) pr.push_back(dsa::PassRange{ps.offset, ps.count, ps.mu, 0});
-  // index of each scheduled link in the per-direction, per-parity link array
-  std::vector<uint32_t> lpos(ns);
-  for (const dsa::Pass &ps : s.S.passes)
-    for (uint32_t q = ps.offset; q < ps.offset + ps.count; ++q) {
-      const uint32_t site = s.S.sites[q];
-      uint32_t par = 0, r = site;
-      for (int d = 0; d < s.G.D; ++d) {
-        par += r % s.G.ext[d];
-        r /= s.G.ext[d];
-      }
-      const uint32_t at = s.G.eo ? ((par & 1u) ? s.G.halfV : 0u) + (site >> 1) : site;
-      lpos[q] = (uint32_t)ps.mu * s.G.V + at;
-    }
+  const size_t ns = s.lay->S.sites.size();
   if (hipMalloc(&s.links, nl * sizeof(uint16_t)) != hipSuccess || hipMalloc(&s.old, ns * sizeof(uint16_t)) != hipSuccess ||
-      hipMalloc(&s.sites, ns * sizeof(uint32_t)) != hipSuccess || hipMalloc(&s.pos, ns * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&s.dE1, ns * sizeof(long long)) != hipSuccess ||
-      hipMalloc(&s.dE2, ns * sizeof(long long)) != hipSuccess || hipMalloc(&s.nacc, ns) != hipSuccess ||
-      hipMalloc(&s.passes, pr.size() * sizeof(dsa::PassRange)) != hipSuccess)
+      hipMalloc(&s.dE2, ns * sizeof(long long)) != hipSuccess || hipMalloc(&s.nacc, ns) != hipSuccess)
     return cleanup(fail(DSA_E_DEVICE, "chain allocation failed"));
-  if (hipMemcpyAsync(s.sites, s.S.sites.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, L->stream) !=
-          hipSuccess ||
-      hipMemcpyAsync(s.pos, lpos.data(), ns * sizeof(uint32_t), hipMemcpyHostToDevice, L->stream) != hipSuccess ||
-      hipMemcpyAsync(s.passes, pr.data(), pr.size() * sizeof(dsa::PassRange), hipMemcpyHostToDevice, L->stream) !=
-          hipSuccess)
-    return cleanup(fail(DSA_E_DEVICE, "upload failed"));
   hipError_t e;
   int32_t *tmp = nullptr;
   if (init_links) {
@@ -309,7 +413,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
 
   dsa::LlrChain d{};
   d.base.links = s.links;
-  d.base.sites = s.sites;
+  d.base.sites = s.lay->sites;
   d.base.mstate = nullptr;
   d.base.T = T;
   d.base.G = s.G;
@@ -319,8 +423,8 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   d.dE1 = s.dE1;
   d.dE2 = s.dE2;
   d.nacc = s.nacc;
-  d.pos = s.pos;
-  d.passes = s.passes;
+  d.pos = s.lay->pos;
+  d.passes = s.lay->passes;
   d.axis = p->axis;
   d.bg = p->bg;
   d.lo = dsa::to_fix(p->Ewin - p->hw);
@@ -339,7 +443,7 @@ int dsa_llr_add(dsa_llr *L, const dsa_group *gc, const dsa_llr_params *p, const 
   d.NRM = p->NRM;
   d.tol = p->tol;
   d.cap = p->cap ? p->cap : 20000;
-  d.npass = (uint32_t)s.S.passes.size();
+  d.npass = (uint32_t)s.lay->S.passes.size();
   d.cls = cls ? 1 : 0;
   d.field = field ? 1 : 0;
   d.cmscale = L->margin_scale;
