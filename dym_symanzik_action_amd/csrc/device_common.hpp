@@ -428,6 +428,24 @@ __device__ inline int metro_certified(double dc, double M, double u2) {
   return dec;
 }
 
+// The same test in single precision, ahead of the double one: dc and M are
+// floats whose band [dc - M, dc + M] holds the exact change; every bound
+// keeps the double test's slack (2^-16 relative on the exponential covers
+// its float argument, v_exp_f32 and u's rounding to float).  -1 where it
+// cannot decide for sure.
+__device__ inline int metro_certified_f(float dc, float M, float u) {
+  if (dc + M < 0.f) return 1;
+  const float lo = dc - M;
+  if (lo > 0.f) {
+    if (lo > 37.f) return u >= 2e-15f ? 0 : -1;  // exp(-37) < 1e-15
+    if (dc >= 60.f) return -1;
+    const float e = __builtin_amdgcn_exp2f(-dc * 1.44269504f);
+    const float w = 1.02f * M + 0x1p-16f;
+    return u < e * (1.f - w) ? 1 : (u >= e * (1.f + w) ? 0 : -1);
+  }
+  return u < 1.f - (2.02f * M + 0x1p-22f) ? 1 : -1;
+}
+
 // A certain rejection from a lower bound: the exact action change is at least
 // dl - M.  True when exp(-(dl - M)) <= u2 for sure, by the float tier of
 // metro_certified (the same slack); false leaves the hit to the full test.
@@ -465,6 +483,28 @@ __device__ inline int exact_hit(const LinkCtx &C, const double *retr, const TabV
   const double newact = C.beta0 + C.beta1 * e1n + C.beta2 * e2n;
   const double probrat = (newact <= oldact) ? 1.0 : dsa_exp(-(newact - oldact));
   return probrat > u2 ? 1 : 0;
+}
+
+// The hit's products t_k by value, for the out-of-line decision below.
+template <int D>
+struct TermSet {
+  uint32_t v[8 * (D - 1)];
+};
+
+// A hit the float tier left open: the double test on the class-valued
+// change dc (band M), else the element values (exact_hit).  Out of line: it
+// runs for a few hits in 10^7, and inline its registers and code weigh on
+// every hit.  Returns the decision (bit 0) and whether the element values
+// made it (bit 1).
+template <int D>
+__device__ __attribute__((noinline)) int decide_open(const double *retr, const uint16_t *mult, const uint16_t *inv,
+                                                     uint32_t P, double beta0, double beta1, double beta2, double dc,
+                                                     double M, uint32_t b, uint32_t p, TermSet<D> ts, double u2) {
+  const int dec = metro_certified(dc, M, u2);
+  if (dec >= 0) return dec;
+  const TabView T{mult, inv, P};
+  const LinkCtx C{nullptr, nullptr, nullptr, nullptr, 0.0, 0u, beta0, beta1, beta2, 0u, 0u, 0u, 0};
+  return exact_hit<D>(C, retr, T, b, p, ts.v, u2) | 2;
 }
 
 // Class-product form of update_link, with identical results.
@@ -650,21 +690,28 @@ __device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk
   const uint32_t link = S.i * (uint32_t)D + (uint32_t)mu;
   const double b1q = C.beta1 * L::kScale, b2q = C.beta2 * L::kScale, margin = K.margin;
   const double ab2q = fabs(C.beta2) * L::kScale;
+  // the float tier's weights and margin (rounded up)
+  const float b1f = (float)b1q, b2f = (float)b2q, mf = (float)margin * (1.f + 0x1p-22f);
   uint32_t acc = 0;
   U4 w{0, 0, 0, 0};
   for (int h = 0; h < kNHit; ++h) {
-    double u2;
+    double u2;  // the acceptance uniform; uf: the same rounded to float
+    float uf;
     uint32_t jp;
     if (pre) {
       jp = pre->jp[h * pre->stride + pre->l];
       u2 = pre->u2[h * pre->stride + pre->l];
+      uf = (float)u2;
     } else if (C.rng == kRngPhilox) {
       if ((h & 1) == 0) w = philox4x32_10(U4{link, (uint32_t)h >> 1, C.sweep_lo, C.sweep_hi}, C.seed, kKeySweep);
       jp = hit_pool_index((h & 1) ? w.z : w.x, C.npool);
-      u2 = hit_uniform((h & 1) ? w.w : w.y);
+      const uint32_t uw = (h & 1) ? w.w : w.y;
+      u2 = hit_uniform(uw);
+      uf = (float)uw * 0x1p-32f;
     } else {
       jp = (uint32_t)(C.npool_d * minstd_canonical(S.ms));
       u2 = minstd_canonical(S.ms);
+      uf = (float)u2;
     }
     const typename L::Row row = Lk.row(jp, T.P);
     int32_t r1n, m1n;
@@ -680,14 +727,24 @@ __device__ inline uint32_t cls_hits(ClsLink<D> &S, const LinkCtx &C, const L &Lk
     } else {
       cls_sums<D>(Lk, row, S.t, r1n, m1n);
     }
-    // class-valued change of action; the exact one lies within M of it
-    const double dc = b1q * ((double)r1n - (double)S.r1) + b2q * ((double)m1n - (double)S.m1);
-    const double M = margin + 0x1p-50 * fabs(dc);
-    int dec = metro_certified(dc, M, u2);
+    // class-valued change of action; the exact one lies within margin of
+    // it.  In floats first: with exact integer differences d1, d2 (unless
+    // they overflow), p1 + p2 is within 2^-22 (|p1| + |p2|) of it, so the
+    // band [dcf - Mf, dcf + Mf] holds the exact change
+    const int32_t d1 = (int32_t)((uint32_t)r1n - (uint32_t)S.r1), d2 = (int32_t)((uint32_t)m1n - (uint32_t)S.m1);
+    const bool ovf = (((r1n ^ S.r1) & (r1n ^ d1)) | ((m1n ^ S.m1) & (m1n ^ d2))) < 0;
+    const float p1 = b1f * (float)d1, p2 = b2f * (float)d2;
+    int dec = ovf ? -1 : metro_certified_f(p1 + p2, mf + 0x1p-21f * (fabsf(p1) + fabsf(p2)), uf);
     const uint32_t p = C.s_pool[jp];
     if (dec < 0) {
-      dec = exact_hit<D>(C, K.retr, T, S.b, p, S.t, u2);
-      ++nfall;
+      const double dc = b1q * ((double)r1n - (double)S.r1) + b2q * ((double)m1n - (double)S.m1);
+      const double M = margin + 0x1p-50 * fabs(dc);
+      TermSet<D> ts;
+#pragma unroll
+      for (int k = 0; k < NL; ++k) ts.v[k] = S.t[k];
+      const int r = decide_open<D>(K.retr, T.mult, T.inv, T.P, C.beta0, C.beta1, C.beta2, dc, M, S.b, p, ts, u2);
+      dec = r & 1;
+      nfall += (uint32_t)r >> 1;
     }
     if (dec) {
       S.b = T.M(S.b, p);
