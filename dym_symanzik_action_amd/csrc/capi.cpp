@@ -70,6 +70,7 @@ struct dsa_chain {
   bool narrow_ok = false;
   bool field_ok = false;  // the narrow form with GF(4) products fits here
   size_t lds_max = 0;  // LDS bytes per block on the chain's device
+  uint32_t ncu = 0;    // its compute units
   double margin8 = 0, margin16 = 0, margin_scale = 1;
   // the matrix kernel: runs here (the group has matrices, they fit LDS),
   // its decision bound and the bound's growth per accepted hit
@@ -479,6 +480,9 @@ int dsa_chain_create(const dsa_group *gc, const dsa_chain_params *p, dsa_chain *
   if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
     lds_max = 0;
   c->lds_max = (size_t)lds_max;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) ncu = 0;
+  c->ncu = ncu > 0 ? (uint32_t)ncu : 0u;
   if (c->T.mat) {
     c->mat_ok = dsa::sweep_cls_lds_bytes(c->T, dsa::kClsMatrix) <= (size_t)lds_max;
     dsa::mat_margin((int)p->D, p->beta0, p->beta1, p->beta2, g->mat_dev, c->mat_m0, c->mat_macc);
@@ -549,6 +553,7 @@ int dsa_chain_sweep(dsa_chain *c, uint32_t n) {
   A.seed = (uint32_t)c->p.seed;
   A.fall = c->d_fall;
   A.lds_max = c->lds_max;
+  A.ncu = c->ncu;
   // the class form this chain runs (kClsNone: the table kernel)
   const int mode = run_mode(c);
   A.cmargin = (mode == dsa::kClsMatrix ? c->mat_m0 : mode == dsa::kClsWide ? c->margin16 : c->margin8) * c->margin_scale;

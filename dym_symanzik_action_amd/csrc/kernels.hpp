@@ -79,6 +79,8 @@ struct SweepArgs {
   // kClsField: every pass through the field kernel (else passes below the
   // small-pass sizes take the narrow small-pass kernel)
   int field_all;
+  // compute units of the device (host side: the field kernel's grid)
+  uint32_t ncu;
 };
 
 size_t sweep_lds_bytes(const Tables &T);
