@@ -133,7 +133,7 @@ void k_sweep_cls(SweepArgs A) {
     for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride) {
       ClsLink<D> S;
       cls_prepare_field<D, B>(S, C, Lk, A.G, FV, A.mu, A.sites[k]);
-      acc += cls_hits<D, B>(S, C, Lk, K, T, A.mu, nfall, nullptr);
+      acc += cls_hits<D, B, L, true>(S, C, Lk, K, T, A.mu, nfall, nullptr);
     }
   } else {
     for (uint32_t k = xcd_block() * blockDim.x + threadIdx.x; k < A.count; k += stride)
