@@ -86,7 +86,7 @@ struct dsa_llr {
   std::vector<Slot> slots;
   hipStream_t stream = nullptr;         // set-up, sweep ends, copies
   hipStream_t gstream[kGroups] = {};    // each group's passes
-  hipEvent_t fork = nullptr, stagger = nullptr, join[kGroups] = {};
+  hipEvent_t fork = nullptr, stagger[kGroups] = {}, join[kGroups] = {};
   dsa::LlrChain *d_chains = nullptr;
   dsa::LlrState *d_state = nullptr;
   dsa::BatchTask *d_tasks = nullptr;
@@ -122,7 +122,9 @@ int rebuild_tasks(dsa_llr *L) {
   std::vector<uint32_t> mem[kGroups];
   uint64_t load[kGroups] = {};
   for (uint32_t c : act) {
-    const int g = load[1] < load[0] ? 1 : 0;
+    int g = 0;
+    for (int h = 1; h < kGroups; ++h)
+      if (load[h] < load[g]) g = h;
     mem[g].push_back(c);
     load[g] += links_of(c);
   }
@@ -205,23 +207,25 @@ int step(dsa_llr *L, uint32_t n) {
   HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nc, L->d_obs, L->nobs, L->stream));
   for (uint32_t it = 0; it < n; ++it) {
     // each group's 32 (or so) passes on its own stream, after the previous
-    // sweep's end; the second group starts once the first group's first
-    // proposal launch is done, so that from then on each group's decision
-    // launch tends to run beside the other's proposal launch; the sweep ends
-    // once both groups are through
+    // sweep's end; each group starts once the group before it has done its
+    // first proposal launch, so that from then on a group's decision launch
+    // tends to run beside another group's proposal launch; the sweep ends
+    // once every group is through
     HIP_TRY(hipEventRecord(L->fork, L->stream));
-    HIP_TRY(hipStreamWaitEvent(L->gstream[0], L->fork, 0));
-    const bool stagger = !L->pass_off[0].empty() && !L->pass_off[1].empty();
-    if (!stagger) HIP_TRY(hipStreamWaitEvent(L->gstream[1], L->fork, 0));
+    int prev[kGroups], last = -1;  // the non-empty group before each one
+    for (int g = 0; g < kGroups; ++g) {
+      prev[g] = last;
+      if (!L->pass_off[g].empty()) last = g;
+    }
     size_t np = 0;
     for (int g = 0; g < kGroups; ++g) np = std::max(np, L->pass_off[g].size());
     for (size_t q = 0; q < np; ++q)
       for (int g = 0; g < kGroups; ++g) {
         if (q >= L->pass_off[g].size()) continue;
         const dsa::BatchTask *tk = L->d_tasks + L->pass_off[g][q];
-        if (g == 1 && q == 0 && stagger) HIP_TRY(hipStreamWaitEvent(L->gstream[1], L->stagger, 0));
+        if (q == 0) HIP_TRY(hipStreamWaitEvent(L->gstream[g], prev[g] < 0 ? L->fork : L->stagger[prev[g]], 0));
         HIP_TRY(dsa::launch_llr_pass(L->D, L->d_chains, L->d_state, tk, L->pass_n[g][q], L->lds, L->gstream[g]));
-        if (g == 0 && q == 0 && stagger) HIP_TRY(hipEventRecord(L->stagger, L->gstream[0]));
+        if (q == 0) HIP_TRY(hipEventRecord(L->stagger[g], L->gstream[g]));
         HIP_TRY(dsa::launch_llr_decide(L->d_chains, L->d_state, L->d_idx + L->idx_off[g], L->idx_n[g], (uint32_t)q,
                                        L->gstream[g]));
       }
@@ -273,11 +277,11 @@ int dsa_llr_create(uint32_t D, uint32_t max_chains, uint64_t hbm_budget, dsa_llr
     lds_max = 0;
   L->lds_max = (size_t)lds_max;
   bool streams_ok = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) == hipSuccess &&
-                    hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) == hipSuccess &&
-                    hipEventCreateWithFlags(&L->stagger, hipEventDisableTiming) == hipSuccess;
+                    hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) == hipSuccess;
   for (int g = 0; g < kGroups && streams_ok; ++g)
     streams_ok = hipStreamCreateWithFlags(&L->gstream[g], hipStreamNonBlocking) == hipSuccess &&
-                 hipEventCreateWithFlags(&L->join[g], hipEventDisableTiming) == hipSuccess;
+                 hipEventCreateWithFlags(&L->join[g], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&L->stagger[g], hipEventDisableTiming) == hipSuccess;
   if (!streams_ok || hipMalloc(&L->d_chains, max_chains * sizeof(dsa::LlrChain)) != hipSuccess ||
       hipMalloc(&L->d_state, max_chains * sizeof(dsa::LlrState)) != hipSuccess)
     return bail(fail(DSA_E_DEVICE, "allocation failed"));
@@ -305,9 +309,9 @@ void dsa_llr_destroy(dsa_llr *L) {
   for (int g = 0; g < kGroups; ++g) {
     if (L->gstream[g]) (void)hipStreamDestroy(L->gstream[g]);
     if (L->join[g]) (void)hipEventDestroy(L->join[g]);
+    if (L->stagger[g]) (void)hipEventDestroy(L->stagger[g]);
   }
   if (L->fork) (void)hipEventDestroy(L->fork);
-  if (L->stagger) (void)hipEventDestroy(L->stagger);
   if (L->stream) (void)hipStreamDestroy(L->stream);
   delete L;
 }
