@@ -199,3 +199,21 @@ def test_benchmark_workload_class_equals_table(dsa, dgroups, beta, seed, start):
         assert form == (("field" if start == "hot" else "class-narrow") if kernel == "auto" else kernel)
         assert cnt_c == cnt_t
         assert np.array_equal(a_c, a_t)
+
+
+def test_field_two_links_per_lane_equals_table(dsa, dgroups):
+    """40^3 x 48 S(1080), hot, two sweeps: each pass has 384,000 links, more
+    than one 512-lane block per CU can take one per lane and not a multiple
+    of 1024, so the field kernel runs 375 blocks with one or two links per
+    lane (the grid-stride tail).  Its links and counters equal the table
+    kernel's bit for bit."""
+    g = dgroups("mys1080-v4")
+    out = {}
+    for kernel in ("table", "field"):
+        with dsa.Chain(g, 4, 48, 40, 0.0, 3.0, -0.25, 5) as c:
+            c.set_kernel(kernel)
+            say(f"case 40^3x48 hot: {kernel} kernel, 2 sweeps")
+            c.update(2)
+            out[kernel] = (c.a, c.counters())
+    assert out["field"][1] == out["table"][1]
+    assert np.array_equal(out["field"][0], out["table"][0])
