@@ -269,6 +269,57 @@ struct FieldView {
   __device__ inline uint32_t E(uint32_t w) const { return key[gf4_key(w)]; }
 };
 
+// One transverse direction's products of build_t_field: j-th nu, the
+// shared right factor Rb of b, U_mu(x+mu)'s left factor and C = U_mu(x-mu) b
+// given; t8 = {t[2j], t[2j+1], t[NS+6j] .. t[NS+6j+5]}.
+template <int D>
+__device__ inline void build_t_field_dir(const uint16_t *links, const Geom &G, const FieldView &F, const Site<D> &s,
+                                         int mu, int j, const Gf4R &Rb, const Gf4L &umu_p, const Gf4R &C,
+                                         uint32_t *t8) {
+  const int nu = j + (j >= mu ? 1 : 0);
+  const Gf4L n_10 = F.L(U<D>(links, G, s, nu, mu, 1, nu, 0));
+  const Gf4L n_00i = F.Li(U<D>(links, G, s, nu, mu, 0, nu, 0));
+  const Gf4L n_1mi = F.Li(U<D>(links, G, s, nu, mu, 1, nu, -1));
+  const Gf4L n_0m = F.L(U<D>(links, G, s, nu, mu, 0, nu, -1));
+  const Gf4R n_11 = F.R(U<D>(links, G, s, nu, mu, 1, nu, 1));
+  const Gf4R n_01i = F.Ri(U<D>(links, G, s, nu, mu, 0, nu, 1));
+  const Gf4R n_20 = F.R(U<D>(links, G, s, nu, mu, 2, nu, 0));
+  const Gf4R n_m0i = F.Ri(U<D>(links, G, s, nu, mu, -1, nu, 0));
+  const Gf4R n_1Mi = F.Ri(U<D>(links, G, s, nu, mu, 1, nu, -2));
+  const Gf4R n_0M = F.R(U<D>(links, G, s, nu, mu, 0, nu, -2));
+  const Gf4R n_2mi = F.Ri(U<D>(links, G, s, nu, mu, 2, nu, -1));
+  const Gf4R n_mm = F.R(U<D>(links, G, s, nu, mu, -1, nu, -1));
+  const Gf4R m_01i = F.Ri(U<D>(links, G, s, mu, mu, 0, nu, 1));
+  const Gf4R m_0mi = F.Ri(U<D>(links, G, s, mu, mu, 0, nu, -1));
+  const Gf4L m_02i = F.Li(U<D>(links, G, s, mu, mu, 0, nu, 2));
+  const Gf4L m_11i = F.Li(U<D>(links, G, s, mu, mu, 1, nu, 1));
+  const Gf4L m_m1i = F.Li(U<D>(links, G, s, mu, mu, -1, nu, 1));
+  const Gf4L m_0Mi = F.Li(U<D>(links, G, s, mu, mu, 0, nu, -2));
+  const Gf4L m_1mi = F.Li(U<D>(links, G, s, mu, mu, 1, nu, -1));
+  const Gf4L m_mmi = F.Li(U<D>(links, G, s, mu, mu, -1, nu, -1));
+  // level 1: products of two links (A, Bd with b), right or left forms
+  // as their later products need them
+  const Gf4R A = gf4_right(gf4_mul(n_00i, Rb));         // U_nu(x)^-1 b
+  const Gf4R Bd = gf4_right(gf4_mul(n_0m, Rb));         // U_nu(x-nu) b
+  const uint32_t X = gf4_mul(n_10, m_01i);              // plaquette-up prefix
+  const uint32_t Y = gf4_mul(n_1mi, m_0mi);             // plaquette-down prefix
+  const Gf4L XL = gf4_left(X), YL = gf4_left(Y);
+  const Gf4L p0 = gf4_left(gf4_mul(n_10, n_11)), p1 = gf4_left(gf4_mul(umu_p, n_20));
+  const Gf4L p3 = gf4_left(gf4_mul(n_1mi, n_1Mi)), p4 = gf4_left(gf4_mul(umu_p, n_2mi));
+  const Gf4R q0 = gf4_right(gf4_mul(m_02i, n_01i)), q1 = gf4_right(gf4_mul(m_11i, m_01i));
+  const Gf4R q2 = gf4_right(gf4_mul(m_m1i, n_m0i)), q3 = gf4_right(gf4_mul(m_0Mi, n_0M));
+  const Gf4R q4 = gf4_right(gf4_mul(m_1mi, m_0mi)), q5 = gf4_right(gf4_mul(m_mmi, n_mm));
+  // levels 2 and 3
+  t8[0] = F.E(gf4_mul(XL, A));
+  t8[1] = F.E(gf4_mul(YL, Bd));
+  t8[2] = F.E(gf4_mul(gf4_left(gf4_mul(p0, q0)), A));
+  t8[3] = F.E(gf4_mul(gf4_left(gf4_mul(p1, q1)), A));
+  t8[4] = F.E(gf4_mul(gf4_left(gf4_mul(XL, q2)), C));
+  t8[5] = F.E(gf4_mul(gf4_left(gf4_mul(p3, q3)), Bd));
+  t8[6] = F.E(gf4_mul(gf4_left(gf4_mul(p4, q4)), Bd));
+  t8[7] = F.E(gf4_mul(gf4_left(gf4_mul(YL, q5)), C));
+}
+
 // build_t with every product a packed GF(4) matrix product (no memory
 // access; the same 85 products).  Each link's matrix (or its inverse's) is
 // read from LDS and split into the factor its one role needs (left or
@@ -285,48 +336,12 @@ __device__ inline void build_t_field(const uint16_t *links, const Geom &G, const
   const Gf4R C = gf4_right(gf4_mul(F.L(U<D>(links, G, s, mu, mu, -1, mu, 0)), Rb));   // U_mu(x-mu) b
 #pragma unroll
   for (int j = 0; j < D - 1; ++j) {
-    const int nu = j + (j >= mu ? 1 : 0);
-    const Gf4L n_10 = F.L(U<D>(links, G, s, nu, mu, 1, nu, 0));
-    const Gf4L n_00i = F.Li(U<D>(links, G, s, nu, mu, 0, nu, 0));
-    const Gf4L n_1mi = F.Li(U<D>(links, G, s, nu, mu, 1, nu, -1));
-    const Gf4L n_0m = F.L(U<D>(links, G, s, nu, mu, 0, nu, -1));
-    const Gf4R n_11 = F.R(U<D>(links, G, s, nu, mu, 1, nu, 1));
-    const Gf4R n_01i = F.Ri(U<D>(links, G, s, nu, mu, 0, nu, 1));
-    const Gf4R n_20 = F.R(U<D>(links, G, s, nu, mu, 2, nu, 0));
-    const Gf4R n_m0i = F.Ri(U<D>(links, G, s, nu, mu, -1, nu, 0));
-    const Gf4R n_1Mi = F.Ri(U<D>(links, G, s, nu, mu, 1, nu, -2));
-    const Gf4R n_0M = F.R(U<D>(links, G, s, nu, mu, 0, nu, -2));
-    const Gf4R n_2mi = F.Ri(U<D>(links, G, s, nu, mu, 2, nu, -1));
-    const Gf4R n_mm = F.R(U<D>(links, G, s, nu, mu, -1, nu, -1));
-    const Gf4R m_01i = F.Ri(U<D>(links, G, s, mu, mu, 0, nu, 1));
-    const Gf4R m_0mi = F.Ri(U<D>(links, G, s, mu, mu, 0, nu, -1));
-    const Gf4L m_02i = F.Li(U<D>(links, G, s, mu, mu, 0, nu, 2));
-    const Gf4L m_11i = F.Li(U<D>(links, G, s, mu, mu, 1, nu, 1));
-    const Gf4L m_m1i = F.Li(U<D>(links, G, s, mu, mu, -1, nu, 1));
-    const Gf4L m_0Mi = F.Li(U<D>(links, G, s, mu, mu, 0, nu, -2));
-    const Gf4L m_1mi = F.Li(U<D>(links, G, s, mu, mu, 1, nu, -1));
-    const Gf4L m_mmi = F.Li(U<D>(links, G, s, mu, mu, -1, nu, -1));
-    // level 1: products of two links (A, Bd with b), right or left forms
-    // as their later products need them
-    const Gf4R A = gf4_right(gf4_mul(n_00i, Rb));         // U_nu(x)^-1 b
-    const Gf4R Bd = gf4_right(gf4_mul(n_0m, Rb));         // U_nu(x-nu) b
-    const uint32_t X = gf4_mul(n_10, m_01i);              // plaquette-up prefix
-    const uint32_t Y = gf4_mul(n_1mi, m_0mi);             // plaquette-down prefix
-    const Gf4L XL = gf4_left(X), YL = gf4_left(Y);
-    const Gf4L p0 = gf4_left(gf4_mul(n_10, n_11)), p1 = gf4_left(gf4_mul(umu_p, n_20));
-    const Gf4L p3 = gf4_left(gf4_mul(n_1mi, n_1Mi)), p4 = gf4_left(gf4_mul(umu_p, n_2mi));
-    const Gf4R q0 = gf4_right(gf4_mul(m_02i, n_01i)), q1 = gf4_right(gf4_mul(m_11i, m_01i));
-    const Gf4R q2 = gf4_right(gf4_mul(m_m1i, n_m0i)), q3 = gf4_right(gf4_mul(m_0Mi, n_0M));
-    const Gf4R q4 = gf4_right(gf4_mul(m_1mi, m_0mi)), q5 = gf4_right(gf4_mul(m_mmi, n_mm));
-    // levels 2 and 3
-    t[2 * j] = F.E(gf4_mul(XL, A));
-    t[2 * j + 1] = F.E(gf4_mul(YL, Bd));
-    t[NS + 6 * j + 0] = F.E(gf4_mul(gf4_left(gf4_mul(p0, q0)), A));
-    t[NS + 6 * j + 1] = F.E(gf4_mul(gf4_left(gf4_mul(p1, q1)), A));
-    t[NS + 6 * j + 2] = F.E(gf4_mul(gf4_left(gf4_mul(XL, q2)), C));
-    t[NS + 6 * j + 3] = F.E(gf4_mul(gf4_left(gf4_mul(p3, q3)), Bd));
-    t[NS + 6 * j + 4] = F.E(gf4_mul(gf4_left(gf4_mul(p4, q4)), Bd));
-    t[NS + 6 * j + 5] = F.E(gf4_mul(gf4_left(gf4_mul(YL, q5)), C));
+    uint32_t t8[8];
+    build_t_field_dir<D>(links, G, F, s, mu, j, Rb, umu_p, C, t8);
+    t[2 * j] = t8[0];
+    t[2 * j + 1] = t8[1];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) t[NS + 6 * j + k] = t8[2 + k];
   }
 }
 

@@ -238,6 +238,146 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
   const double A1q = A1 * ClsNarrow::kScale, A2q = A2 * ClsNarrow::kScale;
   const uint32_t sw_lo = (uint32_t)S.sweep, sw_hi = (uint32_t)(S.sweep >> 32);
   const double npool_d = (double)np;
+  if (fld && blockDim.x >= 4u * tk.count) {
+    // A small task (a quarter of the block's lanes or fewer): four lanes per
+    // link.  Lane q of a link builds transverse direction q's eight products
+    // (build_t_field_dir), and each hit's class sums are split over the
+    // lanes and summed across them; every lane of the link then holds the
+    // same sums and takes the same decision.  Integer sums do not depend on
+    // the order, and the element-value re-decision gathers all products in
+    // the reference's order first, so the results are those of one lane per
+    // link.
+    const uint32_t t = threadIdx.x >> 2, q = threadIdx.x & 3u;
+    if (t < tk.count) {  // a link's four lanes are all in or all out
+      const uint32_t idx = tk.offset + t;
+      const uint32_t i = gmem(BC.sites)[idx];
+      const int mu = tk.mu;
+      const Site<D> s = decode<D>(G, i);
+      const size_t lp = (size_t)mu * G.V + lpos(G, i, s.par);
+      uint32_t b = gmem(BC.links)[lp];
+      const uint32_t b0 = b;
+      const bool own = q < (uint32_t)(D - 1);  // holds direction q's products
+      const uint32_t lane0 = (threadIdx.x & 63u) & ~3u;
+      const FieldView FV{s_fm, s_fmi, s_fkey};
+      uint32_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (own) {
+        const Gf4R Rb = FV.R(b);
+        const Gf4L umu_p = FV.L(U<D>(BC.links, G, s, mu, mu, 1, mu, 0));
+        const Gf4R Cm = gf4_right(gf4_mul(FV.L(U<D>(BC.links, G, s, mu, mu, -1, mu, 0)), Rb));
+        build_t_field_dir<D>(BC.links, G, FV, s, mu, (int)q, Rb, umu_p, Cm, t8);
+      }
+      // the link's class sums through `row`: this lane's terms (2 plaquette,
+      // 6 rectangle), then summed over the four lanes
+      auto sums4 = [&](ClsNarrow::Row row, int32_t &rc, int32_t &mc) {
+        int32_t x = 0, y = 0;
+        if (own) {
+          uint32_t off[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) off[k] = Lk.idx(row, t8[k]);
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const int32_t v = Lk.at(off[k]);
+            if (k < 2) x += v; else y += v;
+          }
+        }
+        x += __shfl_xor(x, 1);
+        x += __shfl_xor(x, 2);
+        y += __shfl_xor(y, 1);
+        y += __shfl_xor(y, 2);
+        rc = x;
+        mc = y;
+      };
+      uint32_t nacc = 0;
+      long long d1 = 0, d2 = 0;
+      int32_t r1c, m1c;
+      sums4(Lk.row0(), r1c, m1c);
+      for (int h = 0; h < kNHit; ++h) {
+        const double u2 = s_u2[h][t];
+        const uint32_t jp = s_jp[h][t];
+        int32_t r1nc, m1nc;
+        sums4(Lk.row(jp, P), r1nc, m1nc);
+        const double xc = A1q * ((double)r1nc - (double)r1c) + A2q * ((double)m1nc - (double)m1c);
+        const double M = margin + 0x1p-50 * fabs(xc);
+        int dec = drive ? (axis == 3 ? gt0_certified(-xc, M) : gt0_certified(xc, M)) : exp_gt_certified(xc, M, u2);
+        if (dec < 0) {  // decide exactly as the element path does: every product, in order
+          uint32_t tt[NL];
+#pragma unroll
+          for (int jj = 0; jj < D - 1; ++jj) {
+            const int src = (int)lane0 + jj;
+            tt[2 * jj] = (uint32_t)__shfl((int)t8[0], src);
+            tt[2 * jj + 1] = (uint32_t)__shfl((int)t8[1], src);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) tt[NS + 6 * jj + k] = (uint32_t)__shfl((int)t8[2 + k], src);
+          }
+          const uint32_t bnew = T.M(b, s_pool[jp]), bi = T.I(b);
+          double e1 = 0, e2 = 0, e1n = 0, e2n = 0;
+#pragma unroll
+          for (int k = 0; k < NL; ++k) {
+            const double v = s_retr[T.M(T.M(b, tt[k]), bi)], vn = s_retr[T.M(T.M(bnew, tt[k]), bi)];
+            if (k < NS) {
+              e1 += v;
+              e1n += vn;
+            } else {
+              e2 += v;
+              e2n += vn;
+            }
+          }
+          const double dS1 = e1n - e1, dS2 = e2n - e2;
+          bool accept;
+          if (axis == 3) {
+            accept = drive ? gx * dS1 + gy * dS2 < 0.0 : dsa_exp(-a * dS1 - a2 * dS2) > u2;
+          } else {
+            const double dEw = axis == 1 ? dS1 : dS2, dEbg = axis == 1 ? dS2 : dS1;
+            accept = drive ? sign * dEw > 0.0 : dsa_exp(-a * dEw - bg * dEbg) > u2;
+          }
+          dec = accept ? 1 : 0;
+        }
+        if (dec) {
+          const uint32_t p = s_pool[jp];
+          b = T.M(b, p);
+          if (own) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) t8[k] = T.M(t8[k], p);
+          }
+          r1c = r1nc;
+          m1c = m1nc;
+          ++nacc;
+        }
+      }
+      if (b != b0) {  // energy change in fixed point: s_k = t_k b^-1, this lane's terms, then summed
+        const uint32_t bi = T.I(b);
+        long long p1 = 0, p2 = 0, q1 = 0, q2 = 0;
+        if (own) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const uint32_t sk = T.M(t8[k], bi);
+            const long long vp = to_fix(s_retr[T.M(b, sk)]), vq = to_fix(s_retr[T.M(b0, sk)]);
+            if (k < 2) {
+              p1 += vp;
+              q1 += vq;
+            } else {
+              p2 += vp;
+              q2 += vq;
+            }
+          }
+        }
+        d1 = p1 - q1;
+        d2 = p2 - q2;
+        d1 += __shfl_xor(d1, 1);
+        d1 += __shfl_xor(d1, 2);
+        d2 += __shfl_xor(d2, 1);
+        d2 += __shfl_xor(d2, 2);
+      }
+      if (q == 0) {
+        if (b != b0) gmem(BC.links)[lp] = (uint16_t)b;
+        gmem(LC.old)[idx] = (uint16_t)b0;
+        gmem(LC.dE1)[idx] = d1;
+        gmem(LC.dE2)[idx] = d2;
+        gmem(LC.nacc)[idx] = (uint8_t)(nacc | (b != b0 ? (uint32_t)kChanged : 0u));
+      }
+    }
+    return;
+  }
   for (uint32_t t = threadIdx.x; t < tk.count; t += blockDim.x) {
     const uint32_t idx = tk.offset + t;
     const uint32_t i = gmem(BC.sites)[idx];
