@@ -37,6 +37,29 @@ __device__ inline long long uniform64(long long v) {
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
+// The sum over each quad of lanes (4l .. 4l + 3), in every lane of the quad:
+// DPP quad permutes (lane ^ 1, then lane ^ 2), no LDS round trip.  Every
+// lane of a quad must be active.
+__device__ inline int32_t quad_sum(int32_t v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
+  v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
+  return v;
+}
+
+template <int CTRL>
+__device__ inline long long dpp64(long long x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, 0xf, 0xf, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)((unsigned long long)x >> 32), CTRL, 0xf, 0xf, false);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ inline long long quad_sum64(long long v) {
+  v += dpp64<0xB1>(v);
+  v += dpp64<0x4E>(v);
+  return v;
+}
+
 // lane k's value (k uniform over the wave)
 __device__ inline long long readlane64(long long v, uint32_t k) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)k);
@@ -280,12 +303,8 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
             if (k < 2) x += v; else y += v;
           }
         }
-        x += __shfl_xor(x, 1);
-        x += __shfl_xor(x, 2);
-        y += __shfl_xor(y, 1);
-        y += __shfl_xor(y, 2);
-        rc = x;
-        mc = y;
+        rc = quad_sum(x);
+        mc = quad_sum(y);
       };
       uint32_t nacc = 0;
       long long d1 = 0, d2 = 0;
@@ -361,12 +380,8 @@ __global__ __launch_bounds__(kBlock) void k_llr_pass(const LlrChain *chains, Llr
             }
           }
         }
-        d1 = p1 - q1;
-        d2 = p2 - q2;
-        d1 += __shfl_xor(d1, 1);
-        d1 += __shfl_xor(d1, 2);
-        d2 += __shfl_xor(d2, 1);
-        d2 += __shfl_xor(d2, 2);
+        d1 = quad_sum64(p1 - q1);
+        d2 = quad_sum64(p2 - q2);
       }
       if (q == 0) {
         if (b != b0) gmem(BC.links)[lp] = (uint16_t)b;
