@@ -205,9 +205,12 @@ hipError_t launch_llr_pass(int D, const LlrChain *chains, LlrState *st, const Ba
 // one block per chain idx[0 .. n) (chains 0 .. n when idx is null)
 hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n, uint32_t pass,
                              hipStream_t s);
-hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n, hipStream_t s);
-hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, uint32_t n, const ObsTask *tasks,
-                             uint32_t ntasks, hipStream_t s);
+// idx: the chains idx[0..n), or chains 0..n-1 when null; the resync's
+// tasks cover the same chains
+hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n,
+                                hipStream_t s);
+hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n,
+                             const ObsTask *tasks, uint32_t ntasks, hipStream_t s);
 
 // ---- the reference llr's serial chain, replayed (llr_ref.hip) ----
 struct LrefState {

@@ -1012,9 +1012,11 @@ __global__ __launch_bounds__(kDecide) void k_llr_decide(const LlrChain *chains, 
 
 // End of a sweep: drive-in bookkeeping (llr.cpp:223-231) and the
 // Robbins-Monro step a += 12/(W^2 (m+1)) (<E> - Ewin) (llr.cpp:237-249).
-__global__ void k_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+// idx: the chains to end (idx[0..n)), or every chain 0..n-1 when null.
+__global__ void k_llr_sweep_end(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = idx ? idx[i] : i;
   const LlrChain &LC = chains[c];
   LlrState &S = st[c];
   if (!llr_active(S.phase)) return;
@@ -1061,10 +1063,13 @@ __global__ void k_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n
 
 // Recompute S1 = sum of plaquette ReTr and S2 = sum of 2x1 rectangle ReTr
 // (llr.cpp:46-55,139-151, each rectangle once) for the chains that asked
-// (resync 1: measure the drift first; 2: initial, no drift).
-__global__ void k_llr_resync_zero(LlrState *st, uint32_t n) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n && st[c].resync) {
+// (resync 1: measure the drift first; 2: initial, no drift); idx as for
+// k_llr_sweep_end, the sum's tasks covering the same chains.
+__global__ void k_llr_resync_zero(LlrState *st, const uint32_t *idx, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = idx ? idx[i] : i;
+  if (st[c].resync) {
     st[c].r1 = 0;
     st[c].r2 = 0;
   }
@@ -1096,9 +1101,11 @@ __global__ __launch_bounds__(kBlock) void k_llr_resync_sum(const LlrChain *chain
   }
 }
 
-__global__ void k_llr_resync_apply(const LlrChain *chains, LlrState *st, uint32_t n) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n || !st[c].resync) return;
+__global__ void k_llr_resync_apply(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = idx ? idx[i] : i;
+  if (!st[c].resync) return;
   LlrState &S = st[c];
   if (S.resync == 1) {
     const int axis = chains[c].axis;
@@ -1140,16 +1147,19 @@ hipError_t launch_llr_decide(const LlrChain *chains, LlrState *st, const uint32_
   return hipGetLastError();
 }
 
-hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, uint32_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_llr_sweep_end, lanes(n), dim3(64), 0, s, chains, st, n);
+hipError_t launch_llr_sweep_end(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n,
+                                hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_llr_sweep_end, lanes(n), dim3(64), 0, s, chains, st, idx, n);
   return hipGetLastError();
 }
 
-hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, uint32_t n, const ObsTask *tasks,
-                             uint32_t ntasks, hipStream_t s) {
-  hipLaunchKernelGGL(k_llr_resync_zero, lanes(n), dim3(64), 0, s, st, n);
+hipError_t launch_llr_resync(const LlrChain *chains, LlrState *st, const uint32_t *idx, uint32_t n,
+                             const ObsTask *tasks, uint32_t ntasks, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_llr_resync_zero, lanes(n), dim3(64), 0, s, st, idx, n);
   if (ntasks) hipLaunchKernelGGL(k_llr_resync_sum, dim3(ntasks), dim3(kBlock), 0, s, chains, st, tasks);
-  hipLaunchKernelGGL(k_llr_resync_apply, lanes(n), dim3(64), 0, s, chains, st, n);
+  hipLaunchKernelGGL(k_llr_resync_apply, lanes(n), dim3(64), 0, s, chains, st, idx, n);
   return hipGetLastError();
 }
 

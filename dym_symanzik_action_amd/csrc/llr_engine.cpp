@@ -1,7 +1,8 @@
 // llr_engine.cpp -- device-resident LLR chains (include/dsa.h, dsa_llr_*):
 // admission into slots, the active chains split into two groups of about
-// equal links, each group's passes launched on a stream of its own (one
-// proposal launch and one decision launch per pass over the group's chains),
+// equal links, each group's sweeps launched on a stream of its own (one
+// proposal launch and one decision launch per pass over the group's chains,
+// then the group's sweep end and energy resync; the groups meet once a step),
 // the per-chain drive-in / Robbins-Monro state machine on the device, and a
 // continuous-batching serving loop for streams of window requests.
 //
@@ -84,7 +85,7 @@ struct dsa_llr {
   uint32_t cap = 0;
   uint64_t budget = 0, used_bytes = 0;
   std::vector<Slot> slots;
-  hipStream_t stream = nullptr;         // set-up, sweep ends, copies
+  hipStream_t stream = nullptr;         // set-up, a step's first resync, copies
   hipStream_t gstream[kGroups] = {};    // each group's passes
   hipEvent_t fork = nullptr, stagger[kGroups] = {}, join[kGroups] = {};
   dsa::LlrChain *d_chains = nullptr;
@@ -98,6 +99,9 @@ struct dsa_llr {
   uint32_t *d_idx = nullptr;
   size_t idx_cap = 0;
   uint32_t idx_off[kGroups] = {}, idx_n[kGroups] = {};
+  // the energy-resync tasks, group by group: group g's are
+  // d_obs[obs_off[g] .. + obs_n[g]), all of them d_obs[0 .. nobs)
+  uint32_t obs_off[kGroups] = {}, obs_n[kGroups] = {};
   uint32_t nobs = 0;
   size_t lds = 0, lds_max = 0;
   int kernel = DSA_KERNEL_AUTO;  // for chains added from now on
@@ -156,11 +160,14 @@ int rebuild_tasks(dsa_llr *L) {
       L->pass_n[g].push_back((uint32_t)tasks.size() - L->pass_off[g].back());
     }
   }
-  for (uint32_t c = 0; c < L->slots.size(); ++c) {
-    const Slot &s = L->slots[c];
-    if (!s.used) continue;
-    for (uint32_t s0 = 0; s0 < s.G.V; s0 += kObsSites)
-      obs.push_back(dsa::ObsTask{c, s0, std::min(kObsSites, s.G.V - s0)});
+  for (int g = 0; g < kGroups; ++g) {
+    L->obs_off[g] = (uint32_t)obs.size();
+    for (uint32_t c : mem[g]) {
+      const Slot &s = L->slots[c];
+      for (uint32_t s0 = 0; s0 < s.G.V; s0 += kObsSites)
+        obs.push_back(dsa::ObsTask{c, s0, std::min(kObsSites, s.G.V - s0)});
+    }
+    L->obs_n[g] = (uint32_t)obs.size() - L->obs_off[g];
   }
   if (tasks.size() > L->tasks_cap) {
     if (L->d_tasks) HIP_TRY(hipFree(L->d_tasks));
@@ -204,37 +211,44 @@ int step(dsa_llr *L, uint32_t n) {
   if (L->dirty)
     if (int rc = rebuild_tasks(L)) return rc;
   const uint32_t nc = (uint32_t)L->slots.size();
-  HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nc, L->d_obs, L->nobs, L->stream));
+  HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nullptr, nc, L->d_obs, L->nobs, L->stream));
+  HIP_TRY(hipEventRecord(L->fork, L->stream));
+  // Each group runs its n sweeps on its own stream: 32 (or so) passes, each
+  // a proposal and a decision launch, then the sweep's end and the energy
+  // resync of the group's own chains, which no other group's kernel reads
+  // or writes.  A group starts once the group before it has done its first
+  // proposal launch, so that a group's decision launch tends to run beside
+  // another group's proposal launch; the groups meet only at the end.
+  int prev[kGroups], last = -1;  // the non-empty group before each one
+  for (int g = 0; g < kGroups; ++g) {
+    prev[g] = last;
+    if (!L->pass_off[g].empty()) last = g;
+  }
+  size_t np = 0;
+  for (int g = 0; g < kGroups; ++g) np = std::max(np, L->pass_off[g].size());
   for (uint32_t it = 0; it < n; ++it) {
-    // each group's 32 (or so) passes on its own stream, after the previous
-    // sweep's end; each group starts once the group before it has done its
-    // first proposal launch, so that from then on a group's decision launch
-    // tends to run beside another group's proposal launch; the sweep ends
-    // once every group is through
-    HIP_TRY(hipEventRecord(L->fork, L->stream));
-    int prev[kGroups], last = -1;  // the non-empty group before each one
-    for (int g = 0; g < kGroups; ++g) {
-      prev[g] = last;
-      if (!L->pass_off[g].empty()) last = g;
-    }
-    size_t np = 0;
-    for (int g = 0; g < kGroups; ++g) np = std::max(np, L->pass_off[g].size());
     for (size_t q = 0; q < np; ++q)
       for (int g = 0; g < kGroups; ++g) {
         if (q >= L->pass_off[g].size()) continue;
         const dsa::BatchTask *tk = L->d_tasks + L->pass_off[g][q];
-        if (q == 0) HIP_TRY(hipStreamWaitEvent(L->gstream[g], prev[g] < 0 ? L->fork : L->stagger[prev[g]], 0));
+        const bool first = it == 0 && q == 0;
+        if (first) HIP_TRY(hipStreamWaitEvent(L->gstream[g], prev[g] < 0 ? L->fork : L->stagger[prev[g]], 0));
         HIP_TRY(dsa::launch_llr_pass(L->D, L->d_chains, L->d_state, tk, L->pass_n[g][q], L->lds, L->gstream[g]));
-        if (q == 0) HIP_TRY(hipEventRecord(L->stagger[g], L->gstream[g]));
+        if (first) HIP_TRY(hipEventRecord(L->stagger[g], L->gstream[g]));
         HIP_TRY(dsa::launch_llr_decide(L->d_chains, L->d_state, L->d_idx + L->idx_off[g], L->idx_n[g], (uint32_t)q,
                                        L->gstream[g]));
       }
     for (int g = 0; g < kGroups; ++g) {
-      HIP_TRY(hipEventRecord(L->join[g], L->gstream[g]));
-      HIP_TRY(hipStreamWaitEvent(L->stream, L->join[g], 0));
+      if (L->pass_off[g].empty()) continue;
+      const uint32_t *ix = L->d_idx + L->idx_off[g];
+      HIP_TRY(dsa::launch_llr_sweep_end(L->d_chains, L->d_state, ix, L->idx_n[g], L->gstream[g]));
+      HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, ix, L->idx_n[g], L->d_obs + L->obs_off[g],
+                                     L->obs_n[g], L->gstream[g]));
     }
-    HIP_TRY(dsa::launch_llr_sweep_end(L->d_chains, L->d_state, nc, L->stream));
-    HIP_TRY(dsa::launch_llr_resync(L->d_chains, L->d_state, nc, L->d_obs, L->nobs, L->stream));
+  }
+  for (int g = 0; g < kGroups; ++g) {
+    HIP_TRY(hipEventRecord(L->join[g], L->gstream[g]));
+    HIP_TRY(hipStreamWaitEvent(L->stream, L->join[g], 0));
   }
   HIP_TRY(hipStreamSynchronize(L->stream));
   return 0;
