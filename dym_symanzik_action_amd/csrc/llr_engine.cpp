@@ -1,5 +1,5 @@
 // llr_engine.cpp -- device-resident LLR chains (include/dsa.h, dsa_llr_*):
-// admission into slots, the active chains split into two groups of about
+// admission into slots, the active chains split into three groups of about
 // equal links, each group's sweeps launched on a stream of its own (one
 // proposal launch and one decision launch per pass over the group's chains,
 // then the group's sweep end and energy resync; the groups meet once a step),
@@ -7,9 +7,13 @@
 // continuous-batching serving loop for streams of window requests.
 //
 // The decision launch walks each chain's proposals in order with one
-// wavefront per chain, which leaves most of the device idle; with two
-// groups the other group's proposal launch runs beside it.  Chains are
-// independent, so the grouping does not change any chain's results.
+// wavefront per chain, which leaves most of the device idle; with three
+// groups another group's proposal launch runs beside it.  Chains are
+// independent, so the grouping does not change any chain's results.  The
+// engine has no stream besides the groups' (set-up and copies go to the
+// first group's): HIP gives a process a few hardware queues (four by
+// default) and hands them to streams as they are created, and two groups
+// whose streams share a queue wait for each other's launches.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -35,7 +39,7 @@ constexpr uint32_t kTaskLinks = 256;
 constexpr uint32_t kTaskLinksSmall = 64;  // for passes below kSmallPass links in all
 constexpr uint64_t kSmallPass = 32768;
 constexpr uint32_t kObsSites = 1024;
-constexpr int kGroups = 2;
+constexpr int kGroups = 3;
 
 // What every chain of one lattice shape shares: the colour schedule (a
 // 16^4 one takes tens of milliseconds to build and verify on the host), the
@@ -85,7 +89,7 @@ struct dsa_llr {
   uint32_t cap = 0;
   uint64_t budget = 0, used_bytes = 0;
   std::vector<Slot> slots;
-  hipStream_t stream = nullptr;         // set-up, a step's first resync, copies
+  hipStream_t stream = nullptr;         // = gstream[0]: set-up, a step's first resync, copies
   hipStream_t gstream[kGroups] = {};    // each group's passes
   hipEvent_t fork = nullptr, stagger[kGroups] = {}, join[kGroups] = {};
   dsa::LlrChain *d_chains = nullptr;
@@ -290,12 +294,12 @@ int dsa_llr_create(uint32_t D, uint32_t max_chains, uint64_t hbm_budget, dsa_llr
   if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, L->device) != hipSuccess)
     lds_max = 0;
   L->lds_max = (size_t)lds_max;
-  bool streams_ok = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking) == hipSuccess &&
-                    hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) == hipSuccess;
+  bool streams_ok = hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) == hipSuccess;
   for (int g = 0; g < kGroups && streams_ok; ++g)
     streams_ok = hipStreamCreateWithFlags(&L->gstream[g], hipStreamNonBlocking) == hipSuccess &&
                  hipEventCreateWithFlags(&L->join[g], hipEventDisableTiming) == hipSuccess &&
                  hipEventCreateWithFlags(&L->stagger[g], hipEventDisableTiming) == hipSuccess;
+  L->stream = L->gstream[0];
   if (!streams_ok || hipMalloc(&L->d_chains, max_chains * sizeof(dsa::LlrChain)) != hipSuccess ||
       hipMalloc(&L->d_state, max_chains * sizeof(dsa::LlrState)) != hipSuccess)
     return bail(fail(DSA_E_DEVICE, "allocation failed"));
@@ -326,7 +330,6 @@ void dsa_llr_destroy(dsa_llr *L) {
     if (L->stagger[g]) (void)hipEventDestroy(L->stagger[g]);
   }
   if (L->fork) (void)hipEventDestroy(L->fork);
-  if (L->stream) (void)hipStreamDestroy(L->stream);
   delete L;
 }
 
